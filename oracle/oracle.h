@@ -1,0 +1,128 @@
+/*
+ * CPU restatement of pRRTC's planning hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker the tests (and the CPU baseline leg of the
+ * benchmark) compare the HIP implementation against.  The product never links
+ * or calls it.  Every function cites the reference file:line it restates.
+ *
+ * Parity status of each part is recorded next to its declaration:
+ *   pinned   -- checked against golden vectors, closed forms or brute force
+ *   unpinned -- no reference output exists to check it against
+ */
+#ifndef PRRTC_ORACLE_H
+#define PRRTC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_DOF 16
+#define OR_MAX_JOINTS 16
+
+/* Robot model: kinematic tree + collision spheres + check structure.
+ * kind 0 = the reference's 3-DoF point robot (one sphere at q, Robots.hh:190-237);
+ * kind 1 = table-driven model (tables fitted from utils.cuh:336-33131). */
+typedef struct {
+    int kind;
+    int dof;
+    float point_radius;                 /* kind 0 only */
+    float lower[OR_MAX_DOF], span[OR_MAX_DOF];  /* Robots.hh get_s_a / get_s_m */
+    int n_joints;
+    int parent[OR_MAX_JOINTS];          /* -1 = world */
+    int prismatic[OR_MAX_JOINTS];
+    float rot[OR_MAX_JOINTS][9];        /* F_j rotation, row-major */
+    float trans[OR_MAX_JOINTS][3];      /* F_j translation */
+    int n_spheres;
+    const int *frame;                   /* [n_spheres], -1 = world */
+    const float *local;                 /* [n_spheres*3] */
+    const float *radius;                /* [n_spheres] */
+    int n_env;
+    const int *env_gate, *env_direct, *env_leaf_off, *env_leaf;   /* off: [n_env+1] */
+    int n_self;
+    const int *self_ga, *self_gb, *self_direct, *self_pair_off, *self_pa, *self_pb;
+    int n_order;
+    const int *order_kind, *order_idx;  /* 0 = env check, 1 = self check */
+} or_model;
+
+/* Obstacle tables, row-major float32 (see prrtc_amd/scenes.py for layout). */
+typedef struct {
+    const float *spheres;    int n_spheres;     /* [n][4]  */
+    const float *capsules;   int n_capsules;    /* [n][8]  */
+    const float *z_capsules; int n_z_capsules;  /* [n][8]  */
+    const float *cuboids;    int n_cuboids;     /* [n][15] */
+    const float *z_cuboids;  int n_z_cuboids;   /* [n][15] */
+} or_scene;
+
+/* FK: sphere centres [n][n_spheres][3] in binary32.  pinned (golden fk_*). */
+void or_fk(const or_model *m, const float *q, int n, float *centres);
+
+/* Per-pose verdict, 1 = collision-free (utils.cuh:211-242, fkcc order),
+ * margin = min over every sphere test of |distance - radius sum| (metres),
+ * used to classify near-contact poses; margin may be NULL.
+ * counts (may be NULL) receives, per pose, the number of tests the reference
+ * executes in its order with its early exits: [sphere, capsule, z_capsule,
+ * cuboid, z_cuboid, self] -- six ints per pose.  pinned (golden verdict_*). */
+void or_check_poses(const or_model *m, const or_scene *s, const float *q, int n,
+                    uint8_t *free_out, float *margin, int32_t *counts);
+
+/* Edge verdicts: g poses a + (k+1)*((b-a)/g), k = 0..g-1 (pRRTC.cu:430-440).
+ * edges [n][2][dof].  pinned (golden edge_*). */
+void or_validate_edges(const or_model *m, const or_scene *s, const float *edges, int n, int g,
+                       uint8_t *free_out);
+
+/* Halton: first D of the bases 3..59, permuted per slot (slot 0 unpermuted);
+ * incremental radical inverse of pRRTC.cu:81-105.  pinned (closed form). */
+typedef struct { float b[OR_MAX_DOF], n[OR_MAX_DOF], d[OR_MAX_DOF]; } or_halton;
+void or_halton_init(or_halton *h, int dof, uint64_t seed, int slot);
+void or_halton_next(or_halton *h, int dof, float *out);
+
+/* Planner settings (pRRTC_settings.hh:3-17) plus the sampler seed. */
+typedef struct {
+    int max_samples, max_iters, num_new_configs, granularity;
+    float range;
+    int balance;
+    float tree_ratio;
+    int dynamic_domain;
+    float dd_alpha, dd_radius, dd_min_radius;
+    uint64_t seed;
+} or_settings;
+
+typedef struct {
+    int solved, start_tree_size, goal_tree_size, path_length, iters;
+    float cost;
+    int path_nodes;          /* rows written to path_out (goal -> start) */
+    int64_t pose_checks;     /* poses whose verdict was computed */
+    int64_t edge_checks;     /* edges (sub-edges) whose verdict was decided */
+} or_result;
+
+/* pRRTC planner restated on the CPU (pRRTC.cu:305-602, 607-766).
+ * mode 0: round-synchronous -- every slot of a round samples against the trees
+ *         as they were at the round start; new nodes are appended in slot order.
+ *         (The HIP planner implements exactly this.)  unpinned vs the reference
+ *         planner (which cannot run here); pinned against the HIP planner.
+ * mode 1: sequential -- slots take turns, each seeing every earlier insertion
+ *         at once, the extreme of the reference's concurrent visibility.
+ * path_out holds up to path_cap rows of dof floats. */
+int or_plan(const or_model *m, const or_scene *s, const float *start, const float *goals, int n_goals,
+            const or_settings *st, int mode, float *path_out, int path_cap, or_result *res);
+
+/* Textbook RRT-Connect (Kuffner & LaValle 2000) with the same range and
+ * per-edge granularity; uniform samples from SplitMix64(seed).  CPU baseline. */
+int or_rrt_connect(const or_model *m, const or_scene *s, const float *start, const float *goal,
+                   int granularity, float range, int max_samples, uint64_t seed,
+                   float *path_out, int path_cap, or_result *res);
+
+/* Re-validate a path edge by edge: poses p_i + (k/g)(p_{i+1}-p_i), k = 0..g.
+ * Returns the index of the first bad edge, or -1 if every edge is free. */
+int or_revalidate_path(const or_model *m, const or_scene *s, const float *path, int n_nodes, int g);
+
+/* Run or_validate_edges over `threads` POSIX threads (static partition). */
+void or_validate_edges_mt(const or_model *m, const or_scene *s, const float *edges, int n, int g,
+                          uint8_t *free_out, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
