@@ -1,0 +1,149 @@
+/*
+ * prrtc -- C ABI of the MI355X (gfx950) pRRTC planning hot path.
+ *
+ * Plain C types only: float/int pointers and sizes.  Every function returns
+ * PRRTC_OK (0) or a negative PRRTC_E* code and never exits the process;
+ * prrtc_last_error() gives the message of the calling thread's last failure.
+ *
+ * Each entry point names the reference interface it replaces
+ * (pranavjad/pRRTC, paths relative to the repository root).
+ */
+#ifndef PRRTC_H
+#define PRRTC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRRTC_OK 0
+#define PRRTC_EINVAL -1      /* bad argument (robot id, size, NULL pointer, NaN) */
+#define PRRTC_EDEVICE -2     /* HIP runtime error */
+#define PRRTC_ENOMEM -3      /* device allocation / admission budget exceeded */
+#define PRRTC_EOVERFLOW -4   /* a caller buffer (path, result) is too small */
+#define PRRTC_EBUSY -5       /* server queue full */
+
+/* Robot ids: the reference's explicit instantiations (src/planning/pRRTC.cu:768-771). */
+#define PRRTC_ROBOT_SPHERE 0 /* 3-DoF point robot, radius 1 (src/planning/Robots.hh:190-237) */
+#define PRRTC_ROBOT_PANDA 1  /* 7 DoF  (Robots.hh:10-64)   */
+#define PRRTC_ROBOT_FETCH 2  /* 8 DoF  (Robots.hh:66-120)  */
+#define PRRTC_ROBOT_BAXTER 3 /* 14 DoF (Robots.hh:122-188) */
+
+/* Obstacle tables, row-major float32.  The five tables the reference's
+ * checker reads (src/planning/utils.cuh:211-242; shapes at
+ * src/collision/shapes.hh:13-245, environment at src/collision/environment.hh:12-45):
+ *   spheres    [n][4]  x y z r
+ *   capsules   [n][8]  x1 y1 z1 xv yv zv r rdv    (rdv = 1/|v|^2; "cylinders" are capsules)
+ *   z_capsules [n][8]  same, v along z
+ *   cuboids    [n][15] x y z a1(xyz) a2(xyz) a3(xyz) r1 r2 r3
+ *   z_cuboids  [n][15] same, a3 = +z                                      */
+typedef struct prrtc_scene {
+    const float *spheres;    int32_t n_spheres;
+    const float *capsules;   int32_t n_capsules;
+    const float *z_capsules; int32_t n_z_capsules;
+    const float *cuboids;    int32_t n_cuboids;
+    const float *z_cuboids;  int32_t n_z_cuboids;
+} prrtc_scene;
+
+/* pRRTC_settings (src/planning/pRRTC_settings.hh:3-17), same fields and
+ * meaning, fixed-width, plus the sampler seed (the reference fixes it to 1,
+ * src/planning/pRRTC.cu:660). */
+typedef struct prrtc_settings {
+    int32_t max_samples;
+    int32_t max_iters;
+    int32_t num_new_configs;
+    int32_t granularity;
+    float range;
+    int32_t balance;
+    float tree_ratio;
+    int32_t dynamic_domain;
+    float dd_alpha;
+    float dd_radius;
+    float dd_min_radius;
+    uint64_t seed;
+} prrtc_settings;
+
+/* PlannerResult (src/planning/Planners.hh:11-23) without the path vector;
+ * the path goes to a caller buffer, goal -> start, as rows of dof floats. */
+typedef struct prrtc_result {
+    int32_t solved;
+    int32_t start_tree_size;
+    int32_t goal_tree_size;
+    int32_t path_length;      /* interior nodes: path_nodes - 2 (pRRTC.cu:743) */
+    int32_t iters;
+    float cost;
+    int32_t path_nodes;       /* rows written to the path buffer */
+    int32_t status;           /* PRRTC_OK or why the problem ended unsolved */
+    int64_t wall_ns;          /* API call -> result (pRRTC.cu:615, 763) */
+    int64_t kernel_ns;        /* device time of the solve (pRRTC.cu:696-706) */
+    int64_t copy_ns;          /* host<->device copies (pRRTC.cu:685-694, 709-713) */
+    int64_t edge_checks;      /* edges whose verdict was decided */
+} prrtc_result;
+
+/* Fill `s` with the reference's defaults (pRRTC_settings.hh:4-16), seed 1. */
+void prrtc_settings_default(prrtc_settings *s);
+
+int prrtc_version(void);
+const char *prrtc_last_error(void);
+int prrtc_robot_dof(int robot);
+int prrtc_robot_num_spheres(int robot);
+
+/* ---- scenes ------------------------------------------------------------ */
+/* Packed device form of a scene (float4-aligned tables, counts in a header). */
+size_t prrtc_scene_packed_floats(const prrtc_scene *scene);
+int prrtc_scene_pack(const prrtc_scene *scene, float *out, size_t cap_floats);
+
+/* ---- FK + collision (replaces ppln::collision::fkcc<Robot>, utils.cuh:326-33131) */
+/* Sphere centres of every collision sphere, [n][num_spheres][3] (debug/parity). */
+int prrtc_fk_spheres(int robot, const float *poses, int n, float *out_centres);
+/* Per-pose verdict, 1 = collision-free. */
+int prrtc_check_poses(int robot, const prrtc_scene *scene, const float *poses, int n, uint8_t *out_free);
+
+/* ---- edge validation (replaces the per-edge block of pRRTC.cu:430-445) ----
+ * edges [n][2][dof]; edge (a,b) is free iff poses a + (k+1)(b-a)/g,
+ * k = 0..g-1, are all free.  Host buffers, synchronous. */
+int prrtc_validate_edges(int robot, const prrtc_scene *scene, const float *edges, int n, int granularity,
+                         uint8_t *out_free);
+/* Same on device buffers, asynchronous on `stream` (a hipStream_t, NULL =
+ * default stream).  d_scene is a packed scene (prrtc_scene_pack) of
+ * scene_floats floats in device memory. */
+int prrtc_validate_edges_async(int robot, const float *d_scene, size_t scene_floats, const float *d_edges, int n,
+                               int granularity, uint8_t *d_out_free, void *stream);
+
+/* ---- planner (replaces pRRTC::solve<Robot>, src/planning/Planners.hh:75-78,
+ *      defined at src/planning/pRRTC.cu:607-766) ----
+ * path: path_cap rows of dof floats, written goal -> start. */
+int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, const prrtc_scene *scene,
+                const prrtc_settings *settings, prrtc_result *result, float *path, int path_cap);
+
+/* Many independent problems in one persistent launch (no reference
+ * counterpart; the reference solves one problem per launch). */
+typedef struct prrtc_problem {
+    int32_t robot;
+    const float *start;       /* dof floats */
+    const float *goals;       /* n_goals * dof floats */
+    int32_t n_goals;
+    prrtc_scene scene;
+} prrtc_problem;
+
+typedef struct prrtc_batch_stats {
+    int64_t wall_ns;          /* whole call */
+    int64_t kernel_ns;        /* device time of the persistent launch */
+    int64_t edge_checks;      /* over all problems */
+    int32_t solved;           /* problems solved */
+    int32_t rounds;           /* planner rounds executed (max over problems) */
+} prrtc_batch_stats;
+
+/* settings[i] applies to problem i (settings may also point to one struct
+ * when n_settings == 1).  paths: n_problems blocks of path_cap rows of 14
+ * floats (rows are dof wide; the block stride is path_cap * 14 floats). */
+int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
+                      int n_settings, prrtc_result *results, float *paths, int path_cap,
+                      prrtc_batch_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRRTC_H */

@@ -1,0 +1,265 @@
+// Device building blocks of the pRRTC hot path for gfx950 (CDNA4, wave64).
+//
+//  * packed scene layout (float4 tables) and its LDS staging;
+//  * sphere-vs-obstacle tests in the reference's squared form
+//    (src/planning/utils.cuh:58-208): value < 0 means collision;
+//  * check contexts that the generated robot code drives, with the
+//    wavefront-level early exit (ballot after every check group);
+//  * robot dispatch (Sphere / Panda / Fetch / Baxter).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "robots/baxter.gen.h"
+#include "robots/fetch.gen.h"
+#include "robots/panda.gen.h"
+
+namespace prrtc {
+
+constexpr int kWave = 64;
+constexpr int kMaxDof = 14;
+// Largest obstacle table set staged into LDS per workgroup.
+constexpr size_t kMaxLdsScene = 64 * 1024;
+
+// ---------------------------------------------------------------------------
+// packed scene: a header of two float4 (int counts), then the tables
+//   spheres  : 1 float4 each   (x, y, z, r)
+//   capsules : 2 float4 each   (x1, y1, z1, r) (xv, yv, zv, rdv)
+//   z-caps   : 2 float4 each
+//   cuboids  : 4 float4 each   (x, y, z, r1) (a1, r2) (a2, r3) (a3, 0)
+//   z-cuboids: 4 float4 each
+struct SceneHeader {
+    int32_t n_sph, n_cap, n_zcap, n_cub, n_zcub, n_f4, pad0, pad1;
+};
+static_assert(sizeof(SceneHeader) == 32, "header is two float4");
+
+__host__ __device__ inline int scene_payload_f4(int n_sph, int n_cap, int n_zcap, int n_cub, int n_zcub) {
+    return n_sph + 2 * (n_cap + n_zcap) + 4 * (n_cub + n_zcub);
+}
+
+// Obstacle tables as seen by one workgroup: counts in registers (uniform),
+// payload behind a pointer (LDS after staging, or global).
+struct Obstacles {
+    const float4 *sph, *cap, *zcap, *cub, *zcub;
+    int n_sph, n_cap, n_zcap, n_cub, n_zcub;
+
+    __device__ void bind(const SceneHeader &h, const float4 *payload) {
+        n_sph = h.n_sph; n_cap = h.n_cap; n_zcap = h.n_zcap; n_cub = h.n_cub; n_zcub = h.n_zcub;
+        sph = payload;
+        cap = sph + n_sph;
+        zcap = cap + 2 * n_cap;
+        cub = zcap + 2 * n_zcap;
+        zcub = cub + 4 * n_cub;
+    }
+};
+
+// Read a packed scene's header.  `cap_f4` is the payload size the host
+// allocated; a header that does not fit it (or is inconsistent) is read as an
+// empty scene, so no access leaves the buffer.
+__device__ inline void read_header(const float *packed, int cap_f4, SceneHeader &h) {
+    const int4 *hdr = reinterpret_cast<const int4 *>(packed);
+    const int4 h0 = hdr[0], h1 = hdr[1];
+    h.n_sph = h0.x; h.n_cap = h0.y; h.n_zcap = h0.z; h.n_cub = h0.w;
+    h.n_zcub = h1.x; h.n_f4 = h1.y; h.pad0 = 0; h.pad1 = 0;
+    const bool ok = h.n_sph >= 0 && h.n_cap >= 0 && h.n_zcap >= 0 && h.n_cub >= 0 && h.n_zcub >= 0 &&
+                    h.n_f4 == scene_payload_f4(h.n_sph, h.n_cap, h.n_zcap, h.n_cub, h.n_zcub) && h.n_f4 <= cap_f4;
+    if (!ok) h = SceneHeader{0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+// Copy a packed scene's payload into LDS; every lane of the workgroup helps.
+__device__ inline void stage_scene(const float *packed, int cap_f4, float4 *lds, SceneHeader &h) {
+    read_header(packed, cap_f4, h);
+    const float4 *src = reinterpret_cast<const float4 *>(packed) + 2;
+    for (int i = threadIdx.x; i < h.n_f4; i += blockDim.x) lds[i] = src[i];
+    __syncthreads();
+}
+
+// Bind a packed scene for a workgroup: staged into LDS, or read in place.
+template <bool LDS>
+__device__ inline void bind_scene(const float *packed, int cap_f4, float4 *lds, Obstacles &obs) {
+    SceneHeader h;
+    if (LDS) {
+        stage_scene(packed, cap_f4, lds, h);
+        obs.bind(h, lds);
+    } else {
+        read_header(packed, cap_f4, h);
+        obs.bind(h, reinterpret_cast<const float4 *>(packed) + 2);
+    }
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// sphere_environment_in_collision (utils.cuh:211-242) for N spheres per lane
+// at once: OR over the five tables, each obstacle read once (an LDS
+// broadcast: every lane reads the same address) and tested against all N
+// spheres.  Tests are the reference's squared forms (utils.cuh:58-208),
+// value < 0 meaning collision.  With EARLY the wavefront leaves the loops as
+// soon as every active lane has a hit in one of its N spheres.
+template <int N, bool EARLY>
+__device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
+                                          const float (&z)[N], const float (&r)[N], bool (&hit)[N]) {
+#pragma unroll
+    for (int n = 0; n < N; n++) hit[n] = false;
+    auto done = [&]() {
+        bool any_hit = false;
+#pragma unroll
+        for (int n = 0; n < N; n++) any_hit |= hit[n];
+        return ballot(!any_hit) == 0;
+    };
+    for (int i = 0; i < o.n_sph; i++) {                       // utils.cuh:58-81
+        const float4 s = o.sph[i];
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const float dx = s.x - x[n], dy = s.y - y[n], dz = s.z - z[n];
+            const float rs = s.w + r[n];
+            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+        }
+        if (EARLY && (i & 7) == 7 && done()) return;
+    }
+    if (EARLY && o.n_sph && done()) return;
+    for (int i = 0; i < o.n_cap; i++) {                       // utils.cuh:113-126
+        const float4 p = o.cap[2 * i], v = o.cap[2 * i + 1];
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const float dot = (x[n] - p.x) * v.x + (y[n] - p.y) * v.y + (z[n] - p.z) * v.z;
+            const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
+            const float dx = x[n] - (p.x + v.x * cdf), dy = y[n] - (p.y + v.y * cdf), dz = z[n] - (p.z + v.z * cdf);
+            const float rs = r[n] + p.w;
+            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+        }
+        if (EARLY && (i & 7) == 7 && done()) return;
+    }
+    if (EARLY && o.n_cap && done()) return;
+    for (int i = 0; i < o.n_zcap; i++) {                      // utils.cuh:135-148
+        const float4 p = o.zcap[2 * i], v = o.zcap[2 * i + 1];
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const float dot = (z[n] - p.z) * v.z;
+            const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
+            const float dx = x[n] - p.x, dy = y[n] - p.y, dz = z[n] - (p.z + v.z * cdf);
+            const float rs = r[n] + p.w;
+            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+        }
+        if (EARLY && (i & 7) == 7 && done()) return;
+    }
+    if (EARLY && o.n_zcap && done()) return;
+    for (int i = 0; i < o.n_cub; i++) {                       // utils.cuh:158-175
+        const float4 c = o.cub[4 * i], a1 = o.cub[4 * i + 1], a2 = o.cub[4 * i + 2], a3 = o.cub[4 * i + 3];
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
+            const float d1 = fmaxf(0.0f, fabsf(a1.x * xs + a1.y * ys + a1.z * zs) - c.w);
+            const float d2 = fmaxf(0.0f, fabsf(a2.x * xs + a2.y * ys + a2.z * zs) - a1.w);
+            const float d3 = fmaxf(0.0f, fabsf(a3.x * xs + a3.y * ys + a3.z * zs) - a2.w);
+            hit[n] |= (d1 * d1 + d2 * d2 + d3 * d3) - r[n] * r[n] < 0.0f;
+        }
+        if (EARLY && (i & 7) == 7 && done()) return;
+    }
+    if (EARLY && o.n_cub && done()) return;
+    for (int i = 0; i < o.n_zcub; i++) {                      // utils.cuh:184-201
+        const float4 c = o.zcub[4 * i], a1 = o.zcub[4 * i + 1], a2 = o.zcub[4 * i + 2];
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
+            const float d1 = fmaxf(0.0f, fabsf(a1.x * xs + a1.y * ys) - c.w);
+            const float d2 = fmaxf(0.0f, fabsf(a2.x * xs + a2.y * ys) - a1.w);
+            const float d3 = fmaxf(0.0f, fabsf(zs) - a2.w);
+            hit[n] |= (d1 * d1 + d2 * d2 + d3 * d3) - r[n] * r[n] < 0.0f;
+        }
+        if (EARLY && (i & 7) == 7 && done()) return;
+    }
+}
+
+// sphere_sphere_self_collision (utils.cuh:83-86) with the radius sum folded.
+__device__ __forceinline__ bool self_hit(float ax, float ay, float az, float bx, float by, float bz, float rs) {
+    const float dx = ax - bx, dy = ay - by, dz = az - bz;
+    return (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Check context driven by the generated robot code, for a pass of up to 64
+// poses (one per lane).
+//
+// Path mode (independent == false): `key` orders the lanes' poses into groups
+// (sub-edges, non-decreasing with the lane index).  Lanes whose key is >= the
+// key of the lowest colliding lane can no longer change the answer, so they
+// stop; the wavefront leaves once no lane is live.  With one key for the
+// whole wave (one edge) the first collision anywhere ends the pass.
+//
+// Pose mode (independent == true): every lane is its own question; a lane
+// stops only on its own collision.
+struct CheckCtx {
+    Obstacles obs;
+    bool coll;
+    bool dead;
+    bool independent;
+    int key;
+
+    __device__ __forceinline__ bool live() const { return !coll && !dead; }
+    __device__ __forceinline__ bool any(bool p) const { return ballot(p) != 0; }
+    template <int N, bool EARLY>
+    __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                              const float (&r)[N], bool (&hit)[N]) const {
+        prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
+    }
+    __device__ __forceinline__ bool self(float ax, float ay, float az, float bx, float by, float bz, float rs) const {
+        return self_hit(ax, ay, az, bx, by, bz, rs);
+    }
+    __device__ __forceinline__ void hit() { coll = true; }
+    __device__ __forceinline__ bool stop() {
+        if (!independent) {
+            const uint64_t cm = ballot(coll);
+            if (cm) {
+                const int l = __builtin_ctzll(cm);
+                const int stop_key = __builtin_amdgcn_readlane(key, l);
+                dead = dead || key >= stop_key;
+            }
+        }
+        return ballot(!coll && !dead) == 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// robots
+
+struct SphereRobot {     // Robots.hh:190-237; fkcc<Sphere>, utils.cuh:330-334
+    static constexpr int DOF = 3;
+    static constexpr int NSPHERES = 1;
+    __host__ __device__ static constexpr float lower(int) { return -5.0f; }
+    __host__ __device__ static constexpr float span(int) { return 10.0f; }
+    template <class Ctx>
+    __device__ __forceinline__ static void check(const float *q, Ctx &ctx) {
+        const float x[1] = {q[0]}, y[1] = {q[1]}, z[1] = {q[2]}, r[1] = {1.0f};
+        bool h[1];
+        if (ctx.live()) {
+            ctx.template env_multi<1, false>(x, y, z, r, h);
+            if (h[0]) ctx.hit();
+        }
+        ctx.stop();
+    }
+    __device__ __forceinline__ static void fk_spheres(const float *q, float *out) {
+        out[0] = q[0]; out[1] = q[1]; out[2] = q[2];
+    }
+};
+
+#define PRRTC_GEN_ROBOT(NAME, NS)                                                              \
+    struct NAME##Robot {                                                                       \
+        static constexpr int DOF = gen::NS::DOF;                                               \
+        static constexpr int NSPHERES = gen::NS::NSPHERES;                                     \
+        __host__ __device__ static constexpr float lower(int i) { return gen::NS::lower(i); } \
+        __host__ __device__ static constexpr float span(int i) { return gen::NS::span(i); }   \
+        template <class Ctx>                                                                   \
+        __device__ __forceinline__ static void check(const float *q, Ctx &ctx) {               \
+            gen::NS::check(q, ctx);                                                            \
+        }                                                                                      \
+        __device__ __forceinline__ static void fk_spheres(const float *q, float *out) {       \
+            gen::NS::fk_spheres(q, out);                                                       \
+        }                                                                                      \
+    };
+PRRTC_GEN_ROBOT(Panda, panda)
+PRRTC_GEN_ROBOT(Fetch, fetch)
+PRRTC_GEN_ROBOT(Baxter, baxter)
+#undef PRRTC_GEN_ROBOT
+
+}  // namespace prrtc

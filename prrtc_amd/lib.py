@@ -1,0 +1,255 @@
+"""ctypes binding of libprrtc.so (the C ABI in include/prrtc.h).
+
+The library must have been built (``python -m prrtc_amd.build``); loading
+fails loudly otherwise -- there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .robots import DOF, ROBOT_IDS
+from .scenes import Scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libprrtc.so")
+
+PRRTC_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUSY"}
+MAX_DOF = 14
+
+_f32p = C.POINTER(C.c_float)
+
+
+class CScene(C.Structure):
+    _fields_ = [
+        ("spheres", _f32p), ("n_spheres", C.c_int32),
+        ("capsules", _f32p), ("n_capsules", C.c_int32),
+        ("z_capsules", _f32p), ("n_z_capsules", C.c_int32),
+        ("cuboids", _f32p), ("n_cuboids", C.c_int32),
+        ("z_cuboids", _f32p), ("n_z_cuboids", C.c_int32),
+    ]
+
+
+class CSettings(C.Structure):
+    _fields_ = [
+        ("max_samples", C.c_int32), ("max_iters", C.c_int32), ("num_new_configs", C.c_int32),
+        ("granularity", C.c_int32), ("range", C.c_float), ("balance", C.c_int32), ("tree_ratio", C.c_float),
+        ("dynamic_domain", C.c_int32), ("dd_alpha", C.c_float), ("dd_radius", C.c_float),
+        ("dd_min_radius", C.c_float), ("seed", C.c_uint64),
+    ]
+
+
+class CResult(C.Structure):
+    _fields_ = [
+        ("solved", C.c_int32), ("start_tree_size", C.c_int32), ("goal_tree_size", C.c_int32),
+        ("path_length", C.c_int32), ("iters", C.c_int32), ("cost", C.c_float), ("path_nodes", C.c_int32),
+        ("status", C.c_int32), ("wall_ns", C.c_int64), ("kernel_ns", C.c_int64), ("copy_ns", C.c_int64),
+        ("edge_checks", C.c_int64),
+    ]
+
+
+class CProblem(C.Structure):
+    _fields_ = [
+        ("robot", C.c_int32), ("start", _f32p), ("goals", _f32p), ("n_goals", C.c_int32), ("scene", CScene),
+    ]
+
+
+class CBatchStats(C.Structure):
+    _fields_ = [
+        ("wall_ns", C.c_int64), ("kernel_ns", C.c_int64), ("edge_checks", C.c_int64), ("solved", C.c_int32),
+        ("rounds", C.c_int32),
+    ]
+
+
+# (name, restype, argtypes) of every symbol include/prrtc.h declares
+SIGNATURES = [
+    ("prrtc_settings_default", None, [C.POINTER(CSettings)]),
+    ("prrtc_version", C.c_int, []),
+    ("prrtc_last_error", C.c_char_p, []),
+    ("prrtc_robot_dof", C.c_int, [C.c_int]),
+    ("prrtc_robot_num_spheres", C.c_int, [C.c_int]),
+    ("prrtc_scene_packed_floats", C.c_size_t, [C.POINTER(CScene)]),
+    ("prrtc_scene_pack", C.c_int, [C.POINTER(CScene), C.c_void_p, C.c_size_t]),
+    ("prrtc_fk_spheres", C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    ("prrtc_check_poses", C.c_int, [C.c_int, C.POINTER(CScene), C.c_void_p, C.c_int, C.c_void_p]),
+    ("prrtc_validate_edges", C.c_int, [C.c_int, C.POINTER(CScene), C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    ("prrtc_validate_edges_async", C.c_int,
+     [C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("prrtc_solve", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(CScene), C.POINTER(CSettings),
+                              C.POINTER(CResult), C.c_void_p, C.c_int]),
+    ("prrtc_solve_batch", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
+                                    C.POINTER(CResult), C.c_void_p, C.c_int, C.POINTER(CBatchStats)]),
+]
+
+_lib = None
+
+
+class PrrtcError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libprrtc.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m prrtc_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != PRRTC_OK:
+        msg = load().prrtc_last_error().decode()
+        raise PrrtcError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+
+def robot_id(robot) -> int:
+    return ROBOT_IDS[robot] if isinstance(robot, str) else int(robot)
+
+
+def _f(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def c_scene(scene: Scene):
+    """A CScene pointing into the scene's arrays (the arrays are kept alive)."""
+    cs = CScene()
+    keep = []
+    for name, w in (("spheres", 4), ("capsules", 8), ("z_capsules", 8), ("cuboids", 15), ("z_cuboids", 15)):
+        a = _f(getattr(scene, name)).reshape(-1, w)
+        keep.append(a)
+        setattr(cs, name, a.ctypes.data_as(_f32p) if len(a) else None)
+        setattr(cs, "n_" + name, len(a))
+    cs._keep = keep
+    return cs
+
+
+def c_settings(d: dict) -> CSettings:
+    st = CSettings()
+    load().prrtc_settings_default(C.byref(st))
+    for k, v in d.items():
+        if not hasattr(st, k):
+            raise KeyError(f"unknown planner setting {k!r}")
+        setattr(st, k, int(v) if k in ("max_samples", "max_iters", "num_new_configs", "granularity", "balance",
+                                        "dynamic_domain", "seed") else float(v))
+    return st
+
+
+def settings_default() -> dict:
+    st = CSettings()
+    load().prrtc_settings_default(C.byref(st))
+    return {name: getattr(st, name) for name, _ in CSettings._fields_}
+
+
+def pack_scene(scene: Scene) -> np.ndarray:
+    cs = c_scene(scene)
+    n = load().prrtc_scene_packed_floats(C.byref(cs))
+    out = np.zeros(n, dtype=np.float32)
+    _check(load().prrtc_scene_pack(C.byref(cs), out.ctypes.data, n), "prrtc_scene_pack")
+    return out
+
+
+def fk_spheres(robot, poses) -> np.ndarray:
+    rid = robot_id(robot)
+    q = _f(poses).reshape(-1, load().prrtc_robot_dof(rid))
+    S = load().prrtc_robot_num_spheres(rid)
+    out = np.zeros((len(q), S, 3), dtype=np.float32)
+    _check(load().prrtc_fk_spheres(rid, q.ctypes.data, len(q), out.ctypes.data), "prrtc_fk_spheres")
+    return out
+
+
+def check_poses(robot, scene: Scene, poses) -> np.ndarray:
+    rid = robot_id(robot)
+    q = _f(poses).reshape(-1, load().prrtc_robot_dof(rid))
+    out = np.zeros(len(q), dtype=np.uint8)
+    cs = c_scene(scene)
+    _check(load().prrtc_check_poses(rid, C.byref(cs), q.ctypes.data, len(q), out.ctypes.data), "prrtc_check_poses")
+    return out.astype(bool)
+
+
+def validate_edges(robot, scene: Scene, edges, granularity: int) -> np.ndarray:
+    rid = robot_id(robot)
+    D = load().prrtc_robot_dof(rid)
+    e = _f(edges).reshape(-1, 2, D)
+    out = np.zeros(len(e), dtype=np.uint8)
+    cs = c_scene(scene)
+    _check(load().prrtc_validate_edges(rid, C.byref(cs), e.ctypes.data, len(e), int(granularity), out.ctypes.data),
+           "prrtc_validate_edges")
+    return out.astype(bool)
+
+
+def validate_edges_async(robot, d_scene: int, scene_floats: int, d_edges: int, n: int, granularity: int,
+                         d_out: int, stream: int = 0) -> None:
+    """Device-pointer form: all pointers are device addresses (ints)."""
+    _check(load().prrtc_validate_edges_async(robot_id(robot), d_scene, scene_floats, d_edges, n, granularity,
+                                             d_out, stream), "prrtc_validate_edges_async")
+
+
+def _result_dict(r: CResult) -> dict:
+    return {name: getattr(r, name) for name, _ in CResult._fields_}
+
+
+def solve(robot, start, goals, scene: Scene, settings: dict | None = None, path_cap: int = 8192):
+    """pRRTC::solve<Robot> (src/planning/Planners.hh:75-78) through the C ABI.
+
+    Returns (result dict, path [rows, dof] goal -> start)."""
+    rid = robot_id(robot)
+    D = load().prrtc_robot_dof(rid)
+    start = _f(start).reshape(D)
+    goals = _f(goals).reshape(-1, D)
+    st = c_settings(settings or {})
+    cs = c_scene(scene)
+    res = CResult()
+    path = np.zeros((path_cap, D), dtype=np.float32)
+    rc = load().prrtc_solve(rid, start.ctypes.data, goals.ctypes.data, len(goals), C.byref(cs), C.byref(st),
+                            C.byref(res), path.ctypes.data, path_cap)
+    _check(rc, "prrtc_solve")
+    return _result_dict(res), path[:res.path_nodes].copy()
+
+
+def solve_batch(problems, settings, path_cap: int = 4096):
+    """Solve many problems in one persistent launch.
+
+    problems: list of dicts {robot, start, goals, scene}; settings: a dict or
+    a list of dicts (one per problem).  Returns (results, paths, stats)."""
+    n = len(problems)
+    cps = (CProblem * n)()
+    keep = []
+    dofs = []
+    for i, p in enumerate(problems):
+        rid = robot_id(p["robot"])
+        D = load().prrtc_robot_dof(rid)
+        dofs.append(D)
+        s = _f(p["start"]).reshape(D)
+        g = _f(p["goals"]).reshape(-1, D)
+        cs = c_scene(p["scene"])
+        keep += [s, g, cs]
+        cps[i].robot = rid
+        cps[i].start = s.ctypes.data_as(_f32p)
+        cps[i].goals = g.ctypes.data_as(_f32p)
+        cps[i].n_goals = len(g)
+        cps[i].scene = cs
+    if isinstance(settings, dict):
+        sts = (CSettings * 1)(c_settings(settings))
+        n_st = 1
+    else:
+        sts = (CSettings * n)(*[c_settings(s) for s in settings])
+        n_st = n
+    res = (CResult * n)()
+    paths = np.zeros((n, path_cap, MAX_DOF), dtype=np.float32)
+    stats = CBatchStats()
+    rc = load().prrtc_solve_batch(n, cps, sts, n_st, res, paths.ctypes.data, path_cap, C.byref(stats))
+    _check(rc, "prrtc_solve_batch")
+    results = [_result_dict(res[i]) for i in range(n)]
+    out_paths = [paths[i, :res[i].path_nodes, :dofs[i]].copy() for i in range(n)]
+    st = {name: getattr(stats, name) for name, _ in CBatchStats._fields_}
+    return results, out_paths, st

@@ -86,6 +86,15 @@ def _snap(x, tol=1e-9):
     return x
 
 
+def _snap_decimal(x, places=5, tol=2e-7):
+    """Lengths in the robot description are given to 0.01 mm or coarser; the
+    fit recovers them to ~1e-7 m.  Round a value to `places` decimals when it
+    lies within `tol` of such a value, keep it otherwise."""
+    x = np.array(x, dtype=np.float64)
+    r = np.round(x, places)
+    return np.where(np.abs(x - r) < tol, r, x)
+
+
 def fit_robot(ref_root: str, robot: str) -> dict:
     prog = F.parse(os.path.join(ref_root, "src/planning/utils.cuh"), robot)
     deps = F.dependencies(prog)
@@ -178,7 +187,7 @@ def fit_robot(ref_root: str, robot: str) -> dict:
         Fr = _snap(Ap.T @ A, 1e-6)
         U, _s, Vt = np.linalg.svd(Fr)
         Fr = _snap(U @ Vt, 1e-12)
-        Ft = _snap(Ap.T @ (point - op), 1e-6)
+        Ft = _snap_decimal(_snap(Ap.T @ (point - op), 1e-6))
         # re-derive this joint's world frame from the snapped F_j
         G[j] = (Ap @ Fr, op + Ap @ Ft)
         joints.append({
@@ -200,7 +209,7 @@ def fit_robot(ref_root: str, robot: str) -> dict:
             frame = cands[0]
         A, o = G[frame]
         local = A.T @ (home[i] - o)
-        out_spheres.append({"frame": int(frame), "local": _snap(local, 1e-12).tolist(), "r": s["r"],
+        out_spheres.append({"frame": int(frame), "local": _snap_decimal(_snap(local, 1e-12)).tolist(), "r": s["r"],
                             "label": s["label"] or ""})
 
     lower, rng = read_limits(os.path.join(ref_root, "src/planning/Robots.hh"), robot)

@@ -1,0 +1,42 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+ROBOTS = ("panda", "fetch", "baxter")
+SCENE_KEYS = ("spheres", "capsules", "z_capsules", "cuboids", "z_cuboids")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def golden_scene(npz, prefix):
+    from prrtc_amd.scenes import Scene
+    return Scene.from_arrays({k: npz[f"{prefix}__{k}"] for k in SCENE_KEYS})
+
+
+@pytest.fixture(scope="session")
+def oracle_robots():
+    from oracle import oracle
+    from prrtc_amd.robots import load_model
+    oracle.build()
+    return {r: oracle.OracleRobot(load_model(r)) for r in ("sphere",) + ROBOTS}
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """The HIP library, loaded; skips nothing -- a missing library is an error."""
+    import prrtc_amd
+    prrtc_amd.load()
+    return prrtc_amd

@@ -1,0 +1,73 @@
+"""The C-ABI library loads and exports every symbol include/prrtc.h declares;
+host-side logic that needs no GPU (settings defaults, scene packing)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "prrtc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(prrtc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import prrtc_amd.lib as L
+    lib = C.CDLL(L.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes binding covers the same set
+    assert sorted(name for name, _, _ in L.SIGNATURES) == syms
+
+
+def test_settings_default_match_reference():
+    import prrtc_amd
+    d = prrtc_amd.settings_default()
+    # pRRTC_settings.hh:4-16
+    assert d["max_samples"] == 1000000 and d["max_iters"] == 1000000
+    assert d["num_new_configs"] == 600 and d["granularity"] == 64
+    assert d["range"] == np.float32(0.5) and d["balance"] == 1 and d["tree_ratio"] == np.float32(0.5)
+    assert d["dynamic_domain"] == 1 and d["dd_alpha"] == np.float32(1e-4)
+    assert d["dd_radius"] == 4.0 and d["dd_min_radius"] == 1.0 and d["seed"] == 1
+
+
+def test_robot_tables():
+    import prrtc_amd
+    L = prrtc_amd.load()
+    assert [L.prrtc_robot_dof(i) for i in range(4)] == [3, 7, 8, 14]
+    assert [L.prrtc_robot_num_spheres(i) for i in range(4)] == [1, 69, 125, 100]
+    assert L.prrtc_robot_dof(7) == -1
+
+
+def test_scene_pack_layout():
+    import prrtc_amd
+    from prrtc_amd import scenes as S
+    sc = S.Scene.build(spheres=[S.sphere((1, 2, 3), 0.5)],
+                       capsules=[S.capsule_endpoints((0, 0, 0), (0, 0, 2), 0.1)],
+                       cuboids=[S.cuboid((1, 1, 1), (0, 0, 0), (0.1, 0.2, 0.3))])
+    p = prrtc_amd.pack_scene(sc)
+    hdr = p[:8].view(np.int32)
+    assert list(hdr[:6]) == [1, 1, 0, 1, 0, 1 + 2 + 4]
+    assert len(p) == 8 + 4 * 7
+    np.testing.assert_array_equal(p[8:12], [1, 2, 3, 0.5])
+    np.testing.assert_allclose(p[12:20], [0, 0, 0, 0.1, 0, 0, 2, 0.25])
+    np.testing.assert_allclose(p[20:36], [1, 1, 1, 0.1, 1, 0, 0, 0.2, 0, 1, 0, 0.3, 0, 0, 1, 0], atol=1e-7)
+
+
+def test_invalid_arguments_fail_without_gpu():
+    """Argument checks run before any device work and return EINVAL."""
+    import prrtc_amd
+    from prrtc_amd.lib import CScene, load
+    L = load()
+    bad = CScene()
+    bad.n_spheres = 3          # count without data
+    rc = L.prrtc_check_poses(1, C.byref(bad), None, 0, None)
+    assert rc == -1
+    assert b"invalid" in L.prrtc_last_error()
+    assert L.prrtc_validate_edges(9, C.byref(CScene()), None, 0, 32, None) == -1
