@@ -24,46 +24,77 @@ static inline float sql2(float ax, float ay, float az, float bx, float by, float
 }
 static inline float clampf(float v, float lo, float hi) { return fmaxf(fminf(v, hi), lo); } /* :50-53 */
 
+/* Rounding: by default the tests are evaluated exactly as written in the
+ * reference (utils.cuh:19-208, separate multiplies and adds).  In "fused"
+ * mode (or_set_fused(1)) they use the explicit fmaf association of the HIP
+ * kernels (prrtc_amd/csrc/device_common.h), so that verdicts -- and planner
+ * runs of the point robot -- can be compared bit for bit. */
+static int g_fused = 0;
+void or_set_fused(int on) { g_fused = on; }
+
+static inline float sq3f(float dx, float dy, float dz) { return fmaf(dx, dx, fmaf(dy, dy, dz * dz)); }
+
 /* each returns the squared-form value (< 0 = collision) and the metric clearance */
 static inline float t_sphere(const float *s, float x, float y, float z, float r, double *clr) {
-    float d = sql2(s[0], s[1], s[2], x, y, z);            /* utils.cuh:58-81 */
+    float d = g_fused ? sq3f(s[0] - x, s[1] - y, s[2] - z)
+                      : sql2(s[0], s[1], s[2], x, y, z);  /* utils.cuh:58-81 */
     float rs = s[3] + r;
     if (clr) *clr = sqrt((double)d) - (double)rs;
-    return d - rs * rs;
+    return g_fused ? fmaf(-rs, rs, d) : d - rs * rs;
 }
 static inline float t_capsule(const float *c, float x, float y, float z, float r, double *clr) {
-    float dot = dot3(x - c[0], y - c[1], z - c[2], c[3], c[4], c[5]);   /* utils.cuh:113-126 */
-    float cdf = clampf(dot * c[7], 0.f, 1.f);
-    float sum = sql2(x, y, z, c[0] + c[3] * cdf, c[1] + c[4] * cdf, c[2] + c[5] * cdf);
+    float dot, cdf, sum;
+    if (g_fused) {
+        dot = fmaf(x - c[0], c[3], fmaf(y - c[1], c[4], (z - c[2]) * c[5]));
+        cdf = clampf(dot * c[7], 0.f, 1.f);
+        sum = sq3f(x - fmaf(c[3], cdf, c[0]), y - fmaf(c[4], cdf, c[1]), z - fmaf(c[5], cdf, c[2]));
+    } else {
+        dot = dot3(x - c[0], y - c[1], z - c[2], c[3], c[4], c[5]);   /* utils.cuh:113-126 */
+        cdf = clampf(dot * c[7], 0.f, 1.f);
+        sum = sql2(x, y, z, c[0] + c[3] * cdf, c[1] + c[4] * cdf, c[2] + c[5] * cdf);
+    }
     float rs = r + c[6];
     if (clr) *clr = sqrt((double)sum) - (double)rs;
-    return sum - rs * rs;
+    return g_fused ? fmaf(-rs, rs, sum) : sum - rs * rs;
 }
 static inline float t_zcapsule(const float *c, float x, float y, float z, float r, double *clr) {
     float dot = (z - c[2]) * c[5];                         /* utils.cuh:135-148 */
     float cdf = clampf(dot * c[7], 0.f, 1.f);
-    float sum = sql2(x, y, z, c[0], c[1], c[2] + c[5] * cdf);
+    float sum = g_fused ? sq3f(x - c[0], y - c[1], z - fmaf(c[5], cdf, c[2]))
+                        : sql2(x, y, z, c[0], c[1], c[2] + c[5] * cdf);
     float rs = r + c[6];
     if (clr) *clr = sqrt((double)sum) - (double)rs;
-    return sum - rs * rs;
+    return g_fused ? fmaf(-rs, rs, sum) : sum - rs * rs;
 }
 static inline float t_cuboid(const float *c, float x, float y, float z, float r, double *clr) {
     float xs = x - c[0], ys = y - c[1], zs = z - c[2];     /* utils.cuh:158-175 */
-    float a1 = fmaxf(0.f, fabsf(dot3(c[3], c[4], c[5], xs, ys, zs)) - c[12]);
-    float a2 = fmaxf(0.f, fabsf(dot3(c[6], c[7], c[8], xs, ys, zs)) - c[13]);
-    float a3 = fmaxf(0.f, fabsf(dot3(c[9], c[10], c[11], xs, ys, zs)) - c[14]);
-    float sum = dot3(a1, a2, a3, a1, a2, a3);
+    float p1, p2, p3;
+    if (g_fused) {
+        p1 = fmaf(c[3], xs, fmaf(c[4], ys, c[5] * zs));
+        p2 = fmaf(c[6], xs, fmaf(c[7], ys, c[8] * zs));
+        p3 = fmaf(c[9], xs, fmaf(c[10], ys, c[11] * zs));
+    } else {
+        p1 = dot3(c[3], c[4], c[5], xs, ys, zs);
+        p2 = dot3(c[6], c[7], c[8], xs, ys, zs);
+        p3 = dot3(c[9], c[10], c[11], xs, ys, zs);
+    }
+    float a1 = fmaxf(0.f, fabsf(p1) - c[12]);
+    float a2 = fmaxf(0.f, fabsf(p2) - c[13]);
+    float a3 = fmaxf(0.f, fabsf(p3) - c[14]);
+    float sum = g_fused ? sq3f(a1, a2, a3) : dot3(a1, a2, a3, a1, a2, a3);
     if (clr) *clr = sqrt((double)sum) - (double)r;
-    return sum - r * r;
+    return g_fused ? fmaf(-r, r, sum) : sum - r * r;
 }
 static inline float t_zcuboid(const float *c, float x, float y, float z, float r, double *clr) {
     float xs = x - c[0], ys = y - c[1], zs = z - c[2];     /* utils.cuh:184-201 */
-    float a1 = fmaxf(0.f, fabsf(c[3] * xs + c[4] * ys) - c[12]);
-    float a2 = fmaxf(0.f, fabsf(c[6] * xs + c[7] * ys) - c[13]);
+    float p1 = g_fused ? fmaf(c[3], xs, c[4] * ys) : c[3] * xs + c[4] * ys;
+    float p2 = g_fused ? fmaf(c[6], xs, c[7] * ys) : c[6] * xs + c[7] * ys;
+    float a1 = fmaxf(0.f, fabsf(p1) - c[12]);
+    float a2 = fmaxf(0.f, fabsf(p2) - c[13]);
     float a3 = fmaxf(0.f, fabsf(zs) - c[14]);
-    float sum = dot3(a1, a2, a3, a1, a2, a3);
+    float sum = g_fused ? sq3f(a1, a2, a3) : dot3(a1, a2, a3, a1, a2, a3);
     if (clr) *clr = sqrt((double)sum) - (double)r;
-    return sum - r * r;
+    return g_fused ? fmaf(-r, r, sum) : sum - r * r;
 }
 
 static inline void upd_margin(double *margin, double clr) {
@@ -110,10 +141,10 @@ static int env_collide(const or_scene *s, float x, float y, float z, float r, in
 
 /* sphere_sphere_self_collision (utils.cuh:83-86) */
 static inline int self_collide(const float *a, float ra, const float *b, float rb, double *margin) {
-    float d = sql2(a[0], a[1], a[2], b[0], b[1], b[2]);
+    float d = g_fused ? sq3f(a[0] - b[0], a[1] - b[1], a[2] - b[2]) : sql2(a[0], a[1], a[2], b[0], b[1], b[2]);
     float rs = ra + rb;
     if (margin) upd_margin(margin, sqrt((double)d) - (double)rs);
-    return d - rs * rs < 0;
+    return (g_fused ? fmaf(-rs, rs, d) : d - rs * rs) < 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -55,6 +55,10 @@ typedef struct {
     const float *z_cuboids;  int n_z_cuboids;   /* [n][15] */
 } or_scene;
 
+/* Select the rounding of the sphere tests: 0 = as written in the reference
+ * (default), 1 = the explicit fmaf association of the HIP kernels. */
+void or_set_fused(int on);
+
 /* FK: sphere centres [n][n_spheres][3] in binary32.  pinned (golden fk_*). */
 void or_fk(const or_model *m, const float *q, int n, float *centres);
 

@@ -92,6 +92,7 @@ def lib():
         L.or_revalidate_path.restype = C.c_int
         L.or_halton_init.argtypes = [vp, C.c_int, C.c_uint64, C.c_int]
         L.or_halton_next.argtypes = [vp, C.c_int, vp]
+        L.or_set_fused.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -249,6 +250,11 @@ def make_settings(d: dict) -> Settings:
     st.dd_min_radius = d.get("dd_min_radius", 1.0)
     st.seed = d.get("seed", 1)
     return st
+
+
+def set_fused(on: bool) -> None:
+    """Sphere tests rounded like the HIP kernels (True) or the reference (False)."""
+    lib().or_set_fused(1 if on else 0)
 
 
 def halton(dof: int, seed: int, slot: int, n: int) -> np.ndarray:

@@ -96,9 +96,16 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 // spheres.  Tests are the reference's squared forms (utils.cuh:58-208),
 // value < 0 meaning collision.  With EARLY the wavefront leaves the loops as
 // soon as every active lane has a hit in one of its N spheres.
+//
+// Rounding is pinned: every multiply-add below is an explicit fmaf in a fixed
+// association (and contraction is off), so the CPU oracle can reproduce each
+// verdict bit for bit in its "fused" mode.
+__device__ __forceinline__ float sq3(float dx, float dy, float dz) { return fmaf(dx, dx, fmaf(dy, dy, dz * dz)); }
+
 template <int N, bool EARLY>
 __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
                                           const float (&z)[N], const float (&r)[N], bool (&hit)[N]) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int n = 0; n < N; n++) hit[n] = false;
     auto done = [&]() {
@@ -111,9 +118,8 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         const float4 s = o.sph[i];
 #pragma unroll
         for (int n = 0; n < N; n++) {
-            const float dx = s.x - x[n], dy = s.y - y[n], dz = s.z - z[n];
             const float rs = s.w + r[n];
-            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+            hit[n] |= fmaf(-rs, rs, sq3(s.x - x[n], s.y - y[n], s.z - z[n])) < 0.0f;
         }
         if (EARLY && (i & 7) == 7 && done()) return;
     }
@@ -122,11 +128,11 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         const float4 p = o.cap[2 * i], v = o.cap[2 * i + 1];
 #pragma unroll
         for (int n = 0; n < N; n++) {
-            const float dot = (x[n] - p.x) * v.x + (y[n] - p.y) * v.y + (z[n] - p.z) * v.z;
+            const float dot = fmaf(x[n] - p.x, v.x, fmaf(y[n] - p.y, v.y, (z[n] - p.z) * v.z));
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
-            const float dx = x[n] - (p.x + v.x * cdf), dy = y[n] - (p.y + v.y * cdf), dz = z[n] - (p.z + v.z * cdf);
             const float rs = r[n] + p.w;
-            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+            const float d = sq3(x[n] - fmaf(v.x, cdf, p.x), y[n] - fmaf(v.y, cdf, p.y), z[n] - fmaf(v.z, cdf, p.z));
+            hit[n] |= fmaf(-rs, rs, d) < 0.0f;
         }
         if (EARLY && (i & 7) == 7 && done()) return;
     }
@@ -137,9 +143,9 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         for (int n = 0; n < N; n++) {
             const float dot = (z[n] - p.z) * v.z;
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
-            const float dx = x[n] - p.x, dy = y[n] - p.y, dz = z[n] - (p.z + v.z * cdf);
             const float rs = r[n] + p.w;
-            hit[n] |= (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+            const float d = sq3(x[n] - p.x, y[n] - p.y, z[n] - fmaf(v.z, cdf, p.z));
+            hit[n] |= fmaf(-rs, rs, d) < 0.0f;
         }
         if (EARLY && (i & 7) == 7 && done()) return;
     }
@@ -149,10 +155,10 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
-            const float d1 = fmaxf(0.0f, fabsf(a1.x * xs + a1.y * ys + a1.z * zs) - c.w);
-            const float d2 = fmaxf(0.0f, fabsf(a2.x * xs + a2.y * ys + a2.z * zs) - a1.w);
-            const float d3 = fmaxf(0.0f, fabsf(a3.x * xs + a3.y * ys + a3.z * zs) - a2.w);
-            hit[n] |= (d1 * d1 + d2 * d2 + d3 * d3) - r[n] * r[n] < 0.0f;
+            const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, fmaf(a1.y, ys, a1.z * zs))) - c.w);
+            const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, fmaf(a2.y, ys, a2.z * zs))) - a1.w);
+            const float d3 = fmaxf(0.0f, fabsf(fmaf(a3.x, xs, fmaf(a3.y, ys, a3.z * zs))) - a2.w);
+            hit[n] |= fmaf(-r[n], r[n], sq3(d1, d2, d3)) < 0.0f;
         }
         if (EARLY && (i & 7) == 7 && done()) return;
     }
@@ -162,10 +168,10 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
-            const float d1 = fmaxf(0.0f, fabsf(a1.x * xs + a1.y * ys) - c.w);
-            const float d2 = fmaxf(0.0f, fabsf(a2.x * xs + a2.y * ys) - a1.w);
+            const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, a1.y * ys)) - c.w);
+            const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, a2.y * ys)) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(zs) - a2.w);
-            hit[n] |= (d1 * d1 + d2 * d2 + d3 * d3) - r[n] * r[n] < 0.0f;
+            hit[n] |= fmaf(-r[n], r[n], sq3(d1, d2, d3)) < 0.0f;
         }
         if (EARLY && (i & 7) == 7 && done()) return;
     }
@@ -173,8 +179,8 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
 
 // sphere_sphere_self_collision (utils.cuh:83-86) with the radius sum folded.
 __device__ __forceinline__ bool self_hit(float ax, float ay, float az, float bx, float by, float bz, float rs) {
-    const float dx = ax - bx, dy = ay - by, dz = az - bz;
-    return (dx * dx + dy * dy + dz * dz) - rs * rs < 0.0f;
+#pragma clang fp contract(off)
+    return fmaf(-rs, rs, sq3(ax - bx, ay - by, az - bz)) < 0.0f;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,5 +1,762 @@
-// Planner entry points (filled in by the round-synchronous planner).
+// Round-synchronous persistent pRRTC planner (gfx950) and its C ABI:
+// prrtc_solve (pRRTC::solve<Robot>, src/planning/Planners.hh:75-78) and
+// prrtc_solve_batch (many problems per launch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "host_common.h"
+#include "planner_device.h"
+
+namespace prrtc {
+
+// ------------------------------------------------------------------ device
+
+// One task: slot s of problem p in the current round (pRRTC.cu:342-551).
+//
+// The steered edge and the connect chain are validated by one loop with a
+// single call site of the robot's check sequence (the sequence is large; one
+// inlined copy keeps register use down).  Uniform vectors live in LDS:
+// chain[i] holds the chain configurations c_0 = new node, c_{i+1} = c_i + vec;
+// uv holds the nearest node, the steered step and the connect step.
+template <class Robot>
+__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx, float *chain, float *uv) {
+#pragma clang fp contract(off)
+    constexpr int D = Robot::DOF;
+    const int lane = lane_id();
+    const int iter = st->round;
+    const int n0 = st->size[0], n1 = st->size[1];
+    const int g = d.g;
+    const float gf = (float)g;
+    int *meta = d.meta + (size_t)s * 8;
+    float *stage = d.stage + (size_t)s * (d.max_ext + 1) * D;
+    float *nnv = uv, *delta = uv + D, *step = uv + 2 * D, *vecv = uv + 3 * D;
+    int edges = 0;
+
+    // tree choice and Halton sample (uniform across the wave)
+    const int t = choose_tree(d, s, iter, d.tprev[s], n0, n1);
+    const int o = 1 - t;
+    float *hs = d.halton + (size_t)s * 3 * D;
+    float sample[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        float b = hs[k], n = hs[D + k], dd = hs[2 * D + k];
+        const float u = halton_step(b, n, dd);
+        if (lane == 0) { hs[D + k] = n; hs[2 * D + k] = dd; }
+        sample[k] = fmaf(u, Robot::span(k), Robot::lower(k));     // Robots.hh scale_cfg
+    }
+    if (lane == 0) d.tprev[s] = t;
+
+    // nearest node in the target tree (pRRTC.cu:384-418)
+    const float *tn = d.nodes + (size_t)t * D * d.cap;
+    float bsq;
+    const int nn = nearest<D>(tn, d.cap, t == 0 ? n0 : n1, sample, bsq);
+    const float dist = sqrtf(bsq);
+    const float scale = fminf(1.0f, d.range / dist);
+    int flags = 0, n_ok = 0, nn_o = 0;
+    if (d.dd && d.radii[(size_t)t * d.cap + nn] < dist) {          // pRRTC.cu:420-427
+        flags = kSkip;
+    } else {
+        if (lane < D) {                                               // pRRTC.cu:430-434
+            const float a = tn[(size_t)lane * d.cap + nn];
+            float sl = sample[0];
+#pragma unroll
+            for (int k = 1; k < D; k++) if (lane == k) sl = sample[k];
+            const float c = fmaf(sl - a, scale, a);
+            nnv[lane] = a;
+            chain[lane] = c;
+            delta[lane] = (c - a) / gf;
+        }
+        __syncthreads();
+        // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
+        // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
+        const int passes = (g + kWave - 1) / kWave;
+        int phase = 0, p = 0, base = 0, total = 0, n_chain = 0;
+        unsigned n_ext = 0;
+        edges = 1;
+        while (true) {
+            float q[D];
+            int key = 0;
+            bool dead;
+            if (phase == 0) {
+                const int k = lane * passes + p;
+                const float kk = (float)((k < g ? k : g - 1) + 1);
+#pragma unroll
+                for (int c = 0; c < D; c++) q[c] = fmaf(kk, delta[c], nnv[c]);
+                dead = k >= g;
+            } else {
+                const int j = base + lane;
+                const int jj = j < total ? j : total - 1;
+                const int i = jj / g, k = jj % g;
+#pragma unroll
+                for (int c = 0; c < D; c++) q[c] = fmaf((float)(k + 1), step[c], chain[i * D + c]);
+                dead = j >= total;
+                key = i;
+            }
+            ctx.coll = false;
+            ctx.dead = dead;
+            ctx.key = key;
+            Robot::check(q, ctx);
+            const uint64_t cm = ballot(ctx.coll);
+            if (phase == 1) {
+                if (cm) {                                             // first colliding sub-edge
+                    n_ok = __builtin_amdgcn_readlane(key, __builtin_ctzll(cm));
+                    break;
+                }
+                base += kWave;
+                if (base >= total) { n_ok = n_chain; break; }
+                continue;
+            }
+            if (cm) break;                                            // steered edge collides
+            if (++p < passes) continue;
+            // steered edge free: connect to the other tree (pRRTC.cu:478-518)
+            flags |= kExtOk;
+            float cfg[D];
+#pragma unroll
+            for (int c = 0; c < D; c++) cfg[c] = chain[c];
+            const float *on = d.nodes + (size_t)o * D * d.cap;
+            nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq);
+            const float dist_o = sqrtf(bsq);
+            n_ext = (unsigned)ceilf(dist_o / d.range);
+            n_chain = (int)min(n_ext, (unsigned)d.max_ext);
+            if (lane < D) {
+                const float vec = (on[(size_t)lane * d.cap + nn_o] - chain[lane]) / (float)n_ext;
+                vecv[lane] = vec;
+                step[lane] = vec / gf;
+            }
+            __syncthreads();
+            // chain configurations by sequential adds, lane i computes c_i
+            for (int i0 = 1; i0 <= n_chain; i0 += kWave) {
+                const int i = i0 + lane;
+                if (i <= n_chain) {
+#pragma unroll
+                    for (int c = 0; c < D; c++) {
+                        const float vc = vecv[c];
+                        float v = chain[c];
+                        for (int r = 0; r < i; r++) v = v + vc;
+                        chain[i * D + c] = v;
+                    }
+                }
+            }
+            __syncthreads();
+            total = n_chain * g;
+            if (total == 0) { n_ok = 0; break; }
+            phase = 1;
+            base = 0;
+        }
+        if (flags & kExtOk) {
+            edges += n_ok < n_chain ? n_ok + 1 : n_ok;
+            if (n_ok == (int)n_ext) flags |= kConnected;
+            // stage the new node and the extension nodes
+            for (int i0 = 0; i0 <= n_ok; i0 += kWave) {
+                const int i = i0 + lane;
+                if (i <= n_ok) {
+#pragma unroll
+                    for (int c = 0; c < D; c++) stage[(size_t)i * D + c] = chain[i * D + c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        meta[0] = t;
+        meta[1] = flags;
+        meta[2] = nn;
+        meta[3] = n_ok;
+        meta[4] = nn_o;
+        __hip_atomic_fetch_add(&st->edge_checks, (unsigned long long)edges, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Walk a parent chain from `cur` to its root; write rows when `out` is set.
+template <int D>
+__device__ int trace(const PlanDesc &d, int t, int cur, float *out, float *cost, int *root) {
+#pragma clang fp contract(off)
+    const float *tn = d.nodes + (size_t)t * D * d.cap;
+    const int *par = d.parent + (size_t)t * d.cap;
+    int n = 0;
+    while (par[cur] != cur) {
+        const int p = par[cur];
+        if (out) {
+            float acc = 0.0f;
+            for (int k = 0; k < D; k++) {
+                const float a = tn[(size_t)k * d.cap + cur];
+                const float diff = a - tn[(size_t)k * d.cap + p];
+                acc = fmaf(diff, diff, acc);
+                out[(size_t)n * D + k] = a;
+            }
+            *cost += sqrtf(acc);
+        }
+        n++;
+        cur = p;
+    }
+    *root = cur;
+    return n;
+}
+
+__device__ __forceinline__ float dd_apply(const PlanDesc &d, float r, bool grow) {
+#pragma clang fp contract(off)
+    if (grow) return r == FLT_MAX ? r : r * (1 + d.dd_alpha);                         // pRRTC.cu:456-468
+    return r == FLT_MAX ? d.dd_radius : fmaxf(r * (1.f - d.dd_alpha), d.dd_min_radius); // pRRTC.cu:584-598
+}
+
+// Commit a finished round (run by the wave that finished its last task).
+// Returns the problem's new status.
+template <class Robot>
+__device__ int plan_commit(const PlanDesc &d, PlanState *st, PlanControl *ctrl, float *paths) {
+    constexpr int D = Robot::DOF;
+    const int lane = lane_id();
+    int sz[2] = {st->size[0], st->size[1]};
+    int status = kStatusRunning;
+    int win_slot = -1, win_last = -1, win_t = 0, win_nn_o = 0;
+    for (int c0 = 0; c0 < d.nnc && status == kStatusRunning && win_slot < 0; c0 += kWave) {
+        const int s = c0 + lane;
+        const bool valid = s < d.nnc;
+        int t = 0, flags = kSkip, nn = 0, n_ok = 0, nn_o = 0;
+        if (valid) {
+            const int *m = d.meta + (size_t)s * 8;
+            t = m[0]; flags = m[1]; nn = m[2]; n_ok = m[3]; nn_o = m[4];
+        }
+        const bool skip = flags & kSkip;
+        const bool ext_ok = flags & kExtOk;
+        const bool conn = ext_ok && (flags & kConnected);
+        const int count = ext_ok ? 1 + n_ok : 0;
+        // positions: exclusive prefix over earlier slots of the same tree
+        int off0 = (t == 0) ? count : 0, off1 = (t == 1) ? count : 0;
+#pragma unroll
+        for (int sh = 1; sh < kWave; sh <<= 1) {
+            const int a0 = __shfl_up(off0, sh), a1 = __shfl_up(off1, sh);
+            if (lane >= sh) { off0 += a0; off1 += a1; }
+        }
+        const int incl = t == 0 ? off0 : off1;
+        const int start = (t == 0 ? sz[0] : sz[1]) + incl - count;
+        const bool over = count > 0 && (start + count > d.max_samples || start + count > d.cap);
+        const uint64_t cm = ballot(valid && conn);
+        const uint64_t om = ballot(valid && over);
+        const int first_conn = cm ? __builtin_ctzll(cm) : kWave;
+        const int first_over = om ? __builtin_ctzll(om) : kWave;
+        const int last_slot = min(first_conn, first_over);       // slots <= last_slot take effect
+        // dynamic-domain updates in slot order (pRRTC.cu:456-468, 584-598):
+        // equal (tree, node) keys are resolved in registers, lane by lane
+        if (d.dd) {
+            const bool has = valid && !skip && lane <= last_slot;
+            const int key = (t << 28) | nn;
+            float r = has ? d.radii[(size_t)t * d.cap + nn] : 0.0f;
+            const uint64_t hm = ballot(has);
+            for (uint64_t m = hm; m; m &= m - 1) {
+                const int j = __builtin_ctzll(m);
+                const int kj = __builtin_amdgcn_readlane(key, j);
+                const int gj = __builtin_amdgcn_readlane((int)ext_ok, j);
+                const float rj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r), j));
+                const float nv = dd_apply(d, rj, gj != 0);
+                if (has && key == kj) r = nv;
+            }
+            if (has) d.radii[(size_t)t * d.cap + nn] = r;
+        }
+        if (first_over <= first_conn && first_over < kWave) {
+            status = kStatusNodeLimit;
+            break;
+        }
+        // insert nodes of slots <= last_slot
+        if (valid && count > 0 && lane <= last_slot) {
+            const float *stage = d.stage + (size_t)s * (d.max_ext + 1) * D;
+            float *tn = d.nodes + (size_t)t * D * d.cap;
+            int *par = d.parent + (size_t)t * d.cap;
+            float *rad = d.radii + (size_t)t * d.cap;
+            for (int i = 0; i < count; i++) {
+                const int idx = start + i;
+#pragma unroll
+                for (int k = 0; k < D; k++) tn[(size_t)k * d.cap + idx] = stage[(size_t)i * D + k];
+                par[idx] = i == 0 ? nn : idx - 1;
+                rad[idx] = FLT_MAX;
+            }
+        }
+        // sizes after this chunk
+        int add0 = __builtin_amdgcn_readlane(off0, min(last_slot, kWave - 1));
+        int add1 = __builtin_amdgcn_readlane(off1, min(last_slot, kWave - 1));
+        sz[0] += add0;
+        sz[1] += add1;
+        if (first_conn < kWave) {
+            win_slot = c0 + first_conn;
+            win_t = __builtin_amdgcn_readlane(t, first_conn);
+            win_nn_o = __builtin_amdgcn_readlane(nn_o, first_conn);
+            const int ws = __builtin_amdgcn_readlane(start, first_conn);
+            const int wc = __builtin_amdgcn_readlane(count, first_conn);
+            win_last = ws + wc - 1;
+        }
+        __threadfence();   // this chunk's radii and nodes before the next chunk reads them
+    }
+    if (status == kStatusRunning && win_slot >= 0) {
+        // trace both trees (pRRTC.cu:552-582) on lane 0
+        if (lane == 0) {
+            st->size[0] = sz[0];
+            st->size[1] = sz[1];
+            float cost = 0.0f;
+            int root_t, root_o;
+            const int tl = trace<D>(d, win_t, win_last, nullptr, &cost, &root_t);
+            const int ol = trace<D>(d, 1 - win_t, win_nn_o, nullptr, &cost, &root_o);
+            const int off = atomic_add_agent(&ctrl->path_rows, tl + ol);
+            if (off + tl + ol > ctrl->path_cap) {
+                st->status = kStatusPathLimit;
+            } else {
+                cost = 0.0f;
+                trace<D>(d, win_t, win_last, paths + (size_t)off * D, &cost, &root_t);
+                trace<D>(d, 1 - win_t, win_nn_o, paths + (size_t)(off + tl) * D, &cost, &root_o);
+                st->t_tree = win_t;
+                st->t_len = tl;
+                st->o_len = ol;
+                st->path_off = off;
+                st->reached = win_t == 1 ? root_t : root_o;
+                st->cost = cost;
+                st->iters = st->round;
+                st->status = kStatusSolved;
+            }
+        }
+        status = __builtin_amdgcn_readfirstlane(lane == 0 ? st->status : 0);
+    } else if (lane == 0) {
+        st->size[0] = sz[0];
+        st->size[1] = sz[1];
+        if (status == kStatusRunning && st->round >= d.max_iters) status = kStatusIterLimit;
+        st->status = status;
+    }
+    status = __builtin_amdgcn_readfirstlane(status);
+    __threadfence();
+    if (lane == 0) {
+        if (status == kStatusRunning) {
+            st->round = st->round + 1;
+            st->done_slots = 0;
+            store_release(&st->next_slot, 0);
+        } else {
+            store_release(&st->next_slot, kClaimedOut);
+            atomic_add_agent(&ctrl->finished, 1);
+        }
+    }
+    return status;
+}
+
+template <class Robot>
+__global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
+                                                PlanControl *ctrl, float *paths, int lds_scene_f4, int lds_ext) {
+    extern __shared__ float4 lds[];
+    float *chain = reinterpret_cast<float *>(lds + lds_scene_f4);
+    float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
+    CheckCtx ctx;
+    ctx.independent = false;
+    int staged = -1;
+    int idle = 0;
+    const int P = ctrl->n_problems;
+    while (true) {
+        if (load_acquire(&ctrl->finished) >= P) break;
+        unsigned tk = 0;
+        if (lane_id() == 0) tk = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tk = __builtin_amdgcn_readfirstlane(tk);
+        const int p = (int)(tk % (unsigned)P);
+        PlanState *st = states + p;
+        const PlanDesc &d = descs[p];
+        // a claim counts only below nnc; finished problems are skipped before
+        // touching their counter, so it never wraps
+        const bool running = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
+        const int s = running ? wave_atomic_add(&st->next_slot, 1) : d.nnc;
+        if (s < 0 || s >= d.nnc) {
+            if (++idle >= P) {
+                __builtin_amdgcn_s_sleep(8);
+                idle = 0;
+            }
+            continue;
+        }
+        idle = 0;
+        if (staged != p) {
+            __syncthreads();
+            bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
+            staged = p;
+        }
+        plan_task<Robot>(d, st, s, ctx, chain, uv);
+        __threadfence();
+        const int prev = wave_atomic_add(&st->done_slots, 1);
+        if (prev == d.nnc - 1) plan_commit<Robot>(d, st, ctrl, paths);
+    }
+}
+
+// Per-problem setup: tree roots, Halton states (pRRTC.cu:644-693).
+template <class Robot>
+__global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict__ descs, PlanState *states) {
+    constexpr int D = Robot::DOF;
+    const PlanDesc &d = descs[blockIdx.x];
+    PlanState *st = states + blockIdx.x;
+    const int lane = lane_id();
+    for (int i = lane; i < 1 + d.n_goals; i += kWave) {
+        const int t = i == 0 ? 0 : 1, idx = i == 0 ? 0 : i - 1;
+        for (int k = 0; k < D; k++) d.nodes[(size_t)t * D * d.cap + (size_t)k * d.cap + idx] = d.init[(size_t)i * D + k];
+        d.parent[(size_t)t * d.cap + idx] = idx;
+        d.radii[(size_t)t * d.cap + idx] = FLT_MAX;
+    }
+    for (int s = lane; s < d.nnc; s += kWave) {
+        float b[16];
+        halton_bases(d.seed, s, D, b);
+        float *hs = d.halton + (size_t)s * 3 * D;
+        for (int k = 0; k < D; k++) { hs[k] = b[k]; hs[D + k] = 0.0f; hs[2 * D + k] = 1.0f; }
+        d.tprev[s] = 0;
+    }
+    if (lane == 0) {
+        *st = PlanState{};
+        st->status = kStatusRunning;
+        st->round = 1;
+        st->size[0] = 1;
+        st->size[1] = d.n_goals;
+        st->next_slot = 0;
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+struct Align {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    }
+};
+
+// Device memory reused across calls of this thread (grown on demand).
+struct Workspace {
+    char *dev = nullptr;
+    size_t bytes = 0;
+    char *host = nullptr;       // pinned staging for uploads / readback
+    size_t host_bytes = 0;
+    char *need_dev(size_t n) {
+        if (n > bytes) {
+            if (dev) (void)hipFree(dev);
+            dev = nullptr;
+            bytes = 0;
+            PRRTC_HIP(hipMalloc(&dev, n));
+            bytes = n;
+        }
+        return dev;
+    }
+    char *need_host(size_t n) {
+        if (n > host_bytes) {
+            if (host) (void)hipHostFree(host);
+            host = nullptr;
+            host_bytes = 0;
+            PRRTC_HIP(hipHostMalloc(&host, n, hipHostMallocDefault));
+            host_bytes = n;
+        }
+        return host;
+    }
+};
+
+static Workspace &workspace() {
+    static thread_local Workspace *ws = new Workspace();   // released with the process
+    return *ws;
+}
+
+struct HostProblem {
+    int robot;
+    const float *start;
+    const float *goals;
+    int n_goals;
+    const prrtc_scene *scene;
+    const prrtc_settings *st;
+    int index;                  // position in the caller's arrays
+};
+
+static int settings_error(const prrtc_settings *s) {
+    if (!s) return 1;
+    if (s->num_new_configs < 1 || s->num_new_configs > kMaxNnc) return 2;
+    if (s->granularity < 1 || s->granularity > 4096) return 3;
+    if (!(s->range > 0.0f) || !std::isfinite(s->range)) return 4;
+    if (s->max_samples < 1 || s->max_iters < 1) return 5;
+    if (s->balance < 0 || s->balance > 2) return 6;
+    if (s->dynamic_domain && (!std::isfinite(s->dd_alpha) || !std::isfinite(s->dd_radius) ||
+                              !std::isfinite(s->dd_min_radius)))
+        return 7;
+    return 0;
+}
+
+static int robot_limits(int robot, float *lo, float *span) {
+    switch (robot) {
+#define PRRTC_LIM(ID, R) \
+    case ID: for (int i = 0; i < R::DOF; i++) { lo[i] = R::lower(i); span[i] = R::span(i); } return R::DOF;
+        PRRTC_LIM(PRRTC_ROBOT_SPHERE, SphereRobot)
+        PRRTC_LIM(PRRTC_ROBOT_PANDA, PandaRobot)
+        PRRTC_LIM(PRRTC_ROBOT_FETCH, FetchRobot)
+        PRRTC_LIM(PRRTC_ROBOT_BAXTER, BaxterRobot)
+#undef PRRTC_LIM
+        default: return -1;
+    }
+}
+
+// Longest possible connect chain: the diagonal of the box spanned by the
+// joint limits, the start and the goals, over `range`.
+static int max_extensions(const HostProblem &hp) {
+    float lo[kMaxDof], span[kMaxDof];
+    const int D = robot_limits(hp.robot, lo, span);
+    double diag = 0.0;
+    for (int k = 0; k < D; k++) {
+        double a = lo[k], b = lo[k] + span[k];
+        a = std::min(a, (double)hp.start[k]);
+        b = std::max(b, (double)hp.start[k]);
+        for (int i = 0; i < hp.n_goals; i++) {
+            a = std::min(a, (double)hp.goals[(size_t)i * D + k]);
+            b = std::max(b, (double)hp.goals[(size_t)i * D + k]);
+        }
+        diag += (b - a) * (b - a);
+    }
+    return (int)std::ceil(std::sqrt(diag) / hp.st->range) + 2;
+}
+
+template <class Robot>
+static void launch_plan(const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths, int P,
+                        int grid, int lds_scene_f4, int max_ext, hipStream_t s) {
+    hipLaunchKernelGGL(k_plan_init<Robot>, dim3(P), dim3(kWave), 0, s, d_desc, d_state);
+    PRRTC_HIP(hipGetLastError());
+    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + (size_t)(max_ext + 6) * Robot::DOF * sizeof(float);
+    hipLaunchKernelGGL(k_plan<Robot>, dim3(grid), dim3(kWave), lds, s, d_desc, d_state, d_ctrl, d_paths,
+                       lds_scene_f4, max_ext);
+    PRRTC_HIP(hipGetLastError());
+}
+
+// Workgroups per CU of the persistent grid (0 = default).
+static int g_wg_per_cu = 0;
+
+// Solve problems that share one robot in one persistent launch.
+static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
+                        int path_stride, int64_t *kernel_ns, int *rounds) {
+    const int P = (int)probs.size();
+    const int robot = probs[0].robot;
+    const int D = robot_dof(robot);
+    Workspace &ws = workspace();
+    // --- layout
+    Align dl;
+    const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
+    const size_t o_state = dl.take(sizeof(PlanState) * P);
+    const size_t o_ctrl = dl.take(sizeof(PlanControl));
+    std::vector<size_t> o_scene(P), o_nodes(P), o_parent(P), o_radii(P), o_halton(P), o_tprev(P), o_meta(P),
+        o_stage(P), o_init(P);
+    std::vector<std::vector<float>> packed(P);
+    std::vector<int> max_ext(P), cap(P);
+    int lds_f4 = 0, ext_all = 0;
+    int64_t path_rows = 0;
+    for (int i = 0; i < P; i++) {
+        const HostProblem &hp = probs[i];
+        packed[i].resize(scene_packed_floats(hp.scene));
+        scene_pack(hp.scene, packed[i].data());
+        lds_f4 = std::max(lds_f4, (int)(packed[i].size() - 8) / 4);
+        max_ext[i] = max_extensions(hp);
+        ext_all = std::max(ext_all, max_ext[i]);
+        cap[i] = std::max(hp.st->max_samples, hp.n_goals);
+        const int nnc = hp.st->num_new_configs;
+        o_scene[i] = dl.take(sizeof(float) * packed[i].size());
+        o_nodes[i] = dl.take(sizeof(float) * 2 * (size_t)D * cap[i]);
+        o_parent[i] = dl.take(sizeof(int) * 2 * (size_t)cap[i]);
+        o_radii[i] = dl.take(sizeof(float) * 2 * (size_t)cap[i]);
+        o_halton[i] = dl.take(sizeof(float) * 3 * (size_t)D * nnc);
+        o_tprev[i] = dl.take(sizeof(int) * nnc);
+        o_meta[i] = dl.take(sizeof(int) * 8 * (size_t)nnc);
+        o_stage[i] = dl.take(sizeof(float) * (size_t)nnc * (max_ext[i] + 1) * D);
+        o_init[i] = dl.take(sizeof(float) * (size_t)(1 + hp.n_goals) * D);
+        path_rows += std::max(0, path_cap - 2);
+    }
+    const size_t o_paths = dl.take(sizeof(float) * (size_t)path_rows * D);
+    char *dev = ws.need_dev(dl.off);
+    // --- host image of everything that is uploaded: descs, control, scenes, init rows
+    const size_t up_bytes = o_ctrl + sizeof(PlanControl);
+    Align hl;
+    const size_t h_hdr = hl.take(up_bytes);
+    std::vector<std::pair<size_t, size_t>> extra;   // (host offset, device offset, via size below)
+    std::vector<size_t> h_scene(P), h_init(P);
+    for (int i = 0; i < P; i++) {
+        h_scene[i] = hl.take(sizeof(float) * packed[i].size());
+        h_init[i] = hl.take(sizeof(float) * (size_t)(1 + probs[i].n_goals) * D);
+    }
+    const size_t h_read = hl.take(sizeof(PlanState) * P + sizeof(PlanControl));
+    char *host = ws.need_host(hl.off);
+    PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + h_hdr + o_desc);
+    for (int i = 0; i < P; i++) {
+        const HostProblem &hp = probs[i];
+        PlanDesc &d = hd[i];
+        d = PlanDesc{};
+        d.nnc = hp.st->num_new_configs;
+        d.g = hp.st->granularity;
+        d.max_iters = hp.st->max_iters;
+        d.max_samples = hp.st->max_samples;
+        d.balance = hp.st->balance;
+        d.dd = hp.st->dynamic_domain ? 1 : 0;
+        d.n_goals = hp.n_goals;
+        d.cap = cap[i];
+        d.max_ext = max_ext[i];
+        d.scene_f4 = (int)(packed[i].size() - 8) / 4;
+        d.range = hp.st->range;
+        d.tree_ratio = hp.st->tree_ratio;
+        d.dd_alpha = hp.st->dd_alpha;
+        d.dd_radius = hp.st->dd_radius;
+        d.dd_min_radius = hp.st->dd_min_radius;
+        d.scene = reinterpret_cast<const float *>(dev + o_scene[i]);
+        d.nodes = reinterpret_cast<float *>(dev + o_nodes[i]);
+        d.parent = reinterpret_cast<int *>(dev + o_parent[i]);
+        d.radii = reinterpret_cast<float *>(dev + o_radii[i]);
+        d.halton = reinterpret_cast<float *>(dev + o_halton[i]);
+        d.tprev = reinterpret_cast<int *>(dev + o_tprev[i]);
+        d.meta = reinterpret_cast<int *>(dev + o_meta[i]);
+        d.stage = reinterpret_cast<float *>(dev + o_stage[i]);
+        d.init = reinterpret_cast<const float *>(dev + o_init[i]);
+        d.seed = hp.st->seed;
+        std::memcpy(host + h_scene[i], packed[i].data(), sizeof(float) * packed[i].size());
+        float *ini = reinterpret_cast<float *>(host + h_init[i]);
+        std::memcpy(ini, hp.start, sizeof(float) * D);
+        std::memcpy(ini + D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D);
+    }
+    PlanControl *hc = reinterpret_cast<PlanControl *>(host + h_hdr + o_ctrl);
+    *hc = PlanControl{};
+    hc->n_problems = P;
+    hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
+    // --- upload (descs + control in one copy; scenes and init rows per problem)
+    hipStream_t s = 0;
+    PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < P; i++) {
+        PRRTC_HIP(hipMemcpyAsync(dev + o_scene[i], host + h_scene[i], sizeof(float) * packed[i].size(),
+                                 hipMemcpyHostToDevice, s));
+        PRRTC_HIP(hipMemcpyAsync(dev + o_init[i], host + h_init[i],
+                                 sizeof(float) * (size_t)(1 + probs[i].n_goals) * D, hipMemcpyHostToDevice, s));
+    }
+    // --- launch
+    int total_slots = 0;
+    for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
+    const int per_cu = g_wg_per_cu > 0 ? g_wg_per_cu : 8;
+    const int grid = std::max(1, std::min(total_slots, num_cus() * per_cu));
+    hipEvent_t e0, e1;
+    PRRTC_HIP(hipEventCreate(&e0));
+    PRRTC_HIP(hipEventCreate(&e1));
+    PRRTC_HIP(hipEventRecord(e0, s));
+    const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
+    PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
+    PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
+    float *d_paths = reinterpret_cast<float *>(dev + o_paths);
+    switch (robot) {
+        case PRRTC_ROBOT_SPHERE: launch_plan<SphereRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
+        case PRRTC_ROBOT_PANDA: launch_plan<PandaRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
+        case PRRTC_ROBOT_FETCH: launch_plan<FetchRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
+        case PRRTC_ROBOT_BAXTER: launch_plan<BaxterRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
+    }
+    PRRTC_HIP(hipEventRecord(e1, s));
+    // --- read back states + control, then the used path rows
+    PlanState *hs = reinterpret_cast<PlanState *>(host + h_read);
+    PRRTC_HIP(hipMemcpyAsync(hs, dev + o_state, sizeof(PlanState) * P + 0, hipMemcpyDeviceToHost, s));
+    PlanControl *hctl = reinterpret_cast<PlanControl *>(host + h_read + sizeof(PlanState) * P);
+    PRRTC_HIP(hipMemcpyAsync(hctl, dev + o_ctrl, sizeof(PlanControl), hipMemcpyDeviceToHost, s));
+    PRRTC_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    PRRTC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *kernel_ns += (int64_t)(ms * 1e6);
+    const int used = std::min(hctl->path_rows, hctl->path_cap);
+    std::vector<float> rows((size_t)used * D);
+    if (used > 0)
+        PRRTC_HIP(hipMemcpy(rows.data(), d_paths, sizeof(float) * rows.size(), hipMemcpyDeviceToHost));
+    // --- results, path goal -> start (pRRTC.cu:715-746)
+    for (int i = 0; i < P; i++) {
+        const HostProblem &hp = probs[i];
+        const PlanState &ps = hs[i];
+        prrtc_result &r = results[hp.index];
+        r.start_tree_size = ps.size[0];
+        r.goal_tree_size = ps.size[1];
+        r.edge_checks = (int64_t)ps.edge_checks;
+        *rounds = std::max(*rounds, ps.round);
+        r.solved = ps.status == kStatusSolved;
+        r.status = r.solved ? PRRTC_OK : (ps.status == kStatusPathLimit ? PRRTC_EOVERFLOW : PRRTC_OK);
+        r.iters = r.solved ? ps.iters : 0;
+        r.cost = r.solved ? ps.cost : 0.0f;
+        r.path_nodes = 0;
+        r.path_length = 0;
+        if (!r.solved) continue;
+        const int n_rows = ps.t_len + ps.o_len + 2;
+        if (n_rows > path_cap) {
+            r.solved = 0;
+            r.status = PRRTC_EOVERFLOW;
+            continue;
+        }
+        float *out = paths + (size_t)hp.index * path_stride;
+        const float *tr = rows.data() + (size_t)ps.path_off * D;      // t-trace then o-trace
+        const float *p1 = ps.t_tree == 1 ? tr : tr + (size_t)ps.t_len * D;
+        const float *p0 = ps.t_tree == 1 ? tr + (size_t)ps.t_len * D : tr;
+        const int n1 = ps.t_tree == 1 ? ps.t_len : ps.o_len;
+        const int n0 = ps.t_tree == 1 ? ps.o_len : ps.t_len;
+        int row = 0;
+        std::memcpy(out + (size_t)row++ * D, hp.goals + (size_t)ps.reached * D, sizeof(float) * D);
+        for (int k = n1 - 1; k >= 0; k--) std::memcpy(out + (size_t)row++ * D, p1 + (size_t)k * D, sizeof(float) * D);
+        for (int k = 0; k < n0; k++) std::memcpy(out + (size_t)row++ * D, p0 + (size_t)k * D, sizeof(float) * D);
+        std::memcpy(out + (size_t)row++ * D, hp.start, sizeof(float) * D);
+        r.path_nodes = row;
+        r.path_length = ps.t_len + ps.o_len;
+    }
+}
+
+static int check_problem(const HostProblem &hp, const char *fn) {
+    const int D = robot_dof(hp.robot);
+    if (D < 0) { set_error("%s: unknown robot id %d", fn, hp.robot); return PRRTC_EINVAL; }
+    if (!hp.start || !hp.goals || hp.n_goals < 1) { set_error("%s: start/goals missing", fn); return PRRTC_EINVAL; }
+    for (int k = 0; k < D; k++)
+        if (!std::isfinite(hp.start[k])) { set_error("%s: non-finite start", fn); return PRRTC_EINVAL; }
+    for (int k = 0; k < D * hp.n_goals; k++)
+        if (!std::isfinite(hp.goals[k])) { set_error("%s: non-finite goal", fn); return PRRTC_EINVAL; }
+    if (!scene_valid(hp.scene)) { set_error("%s: invalid scene", fn); return PRRTC_EINVAL; }
+    if (scene_packed_floats(hp.scene) - 8 > kMaxLdsScene / sizeof(float)) {
+        set_error("%s: scene tables exceed %zu bytes", fn, kMaxLdsScene);
+        return PRRTC_EINVAL;
+    }
+    if (int e = settings_error(hp.st)) { set_error("%s: invalid settings (code %d)", fn, e); return PRRTC_EINVAL; }
+    if (hp.n_goals > hp.st->max_samples) { set_error("%s: more goals than max_samples", fn); return PRRTC_EINVAL; }
+    if (max_extensions(hp) > kMaxExt) {
+        set_error("%s: range %g too small for this robot (connect chains over %d steps)", fn, hp.st->range, kMaxExt);
+        return PRRTC_EINVAL;
+    }
+    return PRRTC_OK;
+}
+
+static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
+                      int path_stride, prrtc_batch_stats *stats, const char *fn) {
+    const int64_t t0 = now_ns();
+    for (auto &hp : probs) {
+        int rc = check_problem(hp, fn);
+        if (rc) return rc;
+    }
+    if (path_cap < 2 || !results || !paths) {
+        set_error("%s: result/path buffers missing or path_cap < 2", fn);
+        return PRRTC_EINVAL;
+    }
+    std::map<int, std::vector<HostProblem>> groups;
+    for (auto &hp : probs) groups[hp.robot].push_back(hp);
+    int64_t kernel_ns = 0;
+    int rounds = 0;
+    for (auto &kv : groups) solve_group(kv.second, results, paths, path_cap, path_stride, &kernel_ns, &rounds);
+    const int64_t wall = now_ns() - t0;
+    int solved = 0;
+    int64_t edges = 0;
+    for (auto &hp : probs) {
+        results[hp.index].wall_ns = wall;
+        results[hp.index].kernel_ns = kernel_ns;
+        results[hp.index].copy_ns = 0;
+        solved += results[hp.index].solved;
+        edges += results[hp.index].edge_checks;
+    }
+    if (stats) {
+        stats->wall_ns = wall;
+        stats->kernel_ns = kernel_ns;
+        stats->edge_checks = edges;
+        stats->solved = solved;
+        stats->rounds = rounds;
+    }
+    return PRRTC_OK;
+}
+
+}  // namespace prrtc
 
 using namespace prrtc;
 
@@ -7,15 +764,33 @@ extern "C" {
 
 int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, const prrtc_scene *scene,
                 const prrtc_settings *settings, prrtc_result *result, float *path, int path_cap) {
-    set_error("prrtc_solve: planner not available in this build");
-    return PRRTC_EINVAL;
+    return guarded("prrtc_solve", [&]() -> int {
+        if (!result) { set_error("prrtc_solve: result is NULL"); return PRRTC_EINVAL; }
+        *result = prrtc_result{};
+        std::vector<HostProblem> probs{HostProblem{robot, start, goals, n_goals, scene, settings, 0}};
+        return solve_many(probs, result, path, path_cap, 0, nullptr, "prrtc_solve");
+    });
 }
 
 int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
                       int n_settings, prrtc_result *results, float *paths, int path_cap,
                       prrtc_batch_stats *stats) {
-    set_error("prrtc_solve_batch: planner not available in this build");
-    return PRRTC_EINVAL;
+    return guarded("prrtc_solve_batch", [&]() -> int {
+        if (n_problems < 0 || (n_problems > 0 && (!problems || !settings)) ||
+            (n_settings != 1 && n_settings != n_problems)) {
+            set_error("prrtc_solve_batch: invalid argument");
+            return PRRTC_EINVAL;
+        }
+        if (stats) *stats = prrtc_batch_stats{};
+        if (n_problems == 0) return PRRTC_OK;
+        std::vector<HostProblem> probs;
+        for (int i = 0; i < n_problems; i++) {
+            results[i] = prrtc_result{};
+            probs.push_back(HostProblem{problems[i].robot, problems[i].start, problems[i].goals, problems[i].n_goals,
+                                        &problems[i].scene, n_settings == 1 ? settings : settings + i, i});
+        }
+        return solve_many(probs, results, paths, path_cap, path_cap * kMaxDof, stats, "prrtc_solve_batch");
+    });
 }
 
 }  // extern "C"

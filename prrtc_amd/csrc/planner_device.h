@@ -1,0 +1,194 @@
+// Round-synchronous pRRTC planner on gfx950: device side.
+//
+// The reference runs num_new_configs (nnc) CUDA blocks per problem, each
+// looping sample -> nearest -> steer -> validate -> insert -> connect with
+// the trees shared through global atomics (src/planning/pRRTC.cu:305-602).
+// Here the same per-sample step is a *task* (problem p, slot s) executed by
+// one wave64, and a problem advances in rounds: every slot of a round works
+// against the trees as committed at the round start; the wave that finishes
+// the round's last task commits the new nodes in slot order, applies the
+// dynamic-domain radius updates in slot order, and detects the solution.
+// Results are therefore a deterministic function of (problem, settings,
+// seed), identical whether a problem runs alone or among hundreds.
+//
+// Persistent kernel: a fixed grid of single-wave workgroups pulls tasks of
+// any running problem from per-problem slot counters; problems of one launch
+// have independent trees, scenes and settings.
+#pragma once
+
+#include "device_common.h"
+
+namespace prrtc {
+
+constexpr int kStatusRunning = 1;
+constexpr int kStatusSolved = 2;
+constexpr int kStatusIterLimit = -1;     // max_iters rounds without a solution
+constexpr int kStatusNodeLimit = -2;     // a tree would exceed max_samples
+constexpr int kStatusPathLimit = -3;     // solution path longer than the buffer
+constexpr int kClaimedOut = 1 << 30;     // next_slot of a finished problem
+constexpr int kMaxExt = 255;             // longest connect chain (sub-edges)
+constexpr int kMaxNnc = 1024;
+
+// Per-problem description: read-only during the launch.
+struct PlanDesc {
+    int nnc, g, max_iters, max_samples;
+    int balance, dd, n_goals, cap;
+    int max_ext, scene_f4, pad0, pad1;
+    float range, tree_ratio, dd_alpha, dd_radius;
+    float dd_min_radius, pad2, pad3, pad4;
+    const float *scene;   // packed scene
+    float *nodes;         // [2][D][cap]   (SoA per tree)
+    int *parent;          // [2][cap]
+    float *radii;         // [2][cap]
+    float *halton;        // [nnc][3][D]   bases, numerators, denominators
+    int *tprev;           // [nnc]         tree each slot extended last round
+    int *meta;            // [nnc][8]      per-slot outcome of the round
+    float *stage;         // [nnc][1+max_ext][D]  new node + extension nodes
+    const float *init;    // start [D] then goals [n_goals][D]
+    uint64_t seed;
+};
+
+// Per-problem mutable state.
+struct PlanState {
+    int status;
+    int round;            // 1-based round (= the reference's per-block iter)
+    int next_slot;        // slot claim counter of the current round
+    int done_slots;       // finished tasks of the current round
+    int size[2];          // committed tree sizes
+    int iters;            // round that found the solution
+    int t_tree;           // tree the connecting slot extended
+    int t_len, o_len;     // path rows traced in each tree
+    int reached;          // goal index reached
+    int path_off;         // first row of this problem's path in the output
+    float cost;
+    int pad0;
+    unsigned long long edge_checks;
+};
+
+// Launch-wide control block.
+struct PlanControl {
+    unsigned int ticket;       // round-robin problem picker
+    int finished;              // problems no longer running
+    int n_problems;
+    int path_rows;             // rows used in the path output
+    int path_cap;              // rows available in the path output
+    int pad[3];
+};
+
+// slot outcome flags in meta[1]
+constexpr int kSkip = 1, kExtOk = 2, kConnected = 4;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+
+__device__ __forceinline__ int atomic_add_agent(int *p, int v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int load_acquire(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lane-0 atomic, result broadcast to the wave.
+__device__ __forceinline__ int wave_atomic_add(int *p, int v) {
+    int r = 0;
+    if (lane_id() == 0) r = atomic_add_agent(p, v);
+    return __builtin_amdgcn_readfirstlane(r);
+}
+
+// ----------------------------------------------------------- planner math
+// All arithmetic below is pinned (contraction off, explicit fmaf where the
+// reference's CUDA build fuses); the CPU oracle's planner uses the same.
+
+// Tree choice of a slot (pRRTC.cu:348-371).
+__device__ __forceinline__ int choose_tree(const PlanDesc &d, int slot, int iter, int t_prev, int n0, int n1) {
+#pragma clang fp contract(off)
+    int t = t_prev;
+    const int nnc = d.nnc;
+    if (d.balance == 0 || iter == 1) {
+        t = (slot < (nnc / 2)) ? 0 : 1;
+    } else if (d.balance == 1 && abs(n0 - n1) < 1.5 * nnc) {
+        const float ratio = n0 / (float)(n0 + n1);
+        const float bf = 1 - ratio;
+        t = (slot < (nnc * bf)) ? 0 : 1;
+    } else if (d.balance == 1) {
+        const float ratio = n0 / (float)(n0 + n1);
+        t = (ratio < d.tree_ratio) ? 0 : 1;
+    } else if (d.balance == 2) {
+        const int nt = t == 0 ? n0 : n1, no = t == 0 ? n1 : n0;
+        const float ratio = abs(nt - no) / (float)nt;
+        if (ratio < d.tree_ratio) t = 1 - t;
+    }
+    return t;
+}
+
+// Incremental Halton step of one dimension (pRRTC.cu:81-105); uniform.
+__device__ __forceinline__ float halton_step(float &b, float &n, float &dd) {
+#pragma clang fp contract(off)
+    const float xf = dd - n;
+    if (xf == 1.0f) {
+        dd = floorf(dd * b);
+        n = 1.0f;
+    } else {
+        float y = floorf(dd / b);
+        // y shrinks by a factor b >= 3 per step; 64 steps bound any float state
+        for (int guard = 0; xf <= y && guard < 64; guard++) y = floorf(y / b);
+        n = floorf((b + 1.0f) * y) - xf;
+    }
+    return n / dd;
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014).
+__host__ __device__ inline uint64_t splitmix64(uint64_t &st) {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Bases of a slot: the first D of {3, 5, ..., 59}, Fisher-Yates shuffled
+// unless slot 0 (pRRTC.cu:49-65); the shuffle draws from SplitMix64 seeded
+// with seed * phi64 + slot * 0xD1B54A32D192ED03.
+__host__ __device__ inline void halton_bases(uint64_t seed, int slot, int dof, float *out) {
+    float primes[16] = {3.f, 5.f, 7.f, 11.f, 13.f, 17.f, 19.f, 23.f, 29.f, 31.f, 37.f, 41.f, 43.f, 47.f, 53.f, 59.f};
+    if (slot != 0) {
+        uint64_t st = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)slot * 0xD1B54A32D192ED03ULL;
+        for (int i = 15; i > 0; i--) {
+            const int j = (int)(splitmix64(st) % (uint64_t)(i + 1));
+            const float tmp = primes[i];
+            primes[i] = primes[j];
+            primes[j] = tmp;
+        }
+    }
+    for (int i = 0; i < dof; i++) out[i] = primes[i];
+}
+
+// Nearest node of tree t to q among [0, n): squared L2 as an fmaf chain in
+// dimension order (utils.cuh:100-109), ties to the smaller index.  The wave
+// strides the SoA coordinates (coalesced) and reduces by lane shuffles.
+template <int D>
+__device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq) {
+#pragma clang fp contract(off)
+    float best = __builtin_huge_valf();
+    int idx = 0x7fffffff;
+    for (int i = lane_id(); i < n; i += kWave) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const float diff = tree[(size_t)d * cap + i] - q[d];
+            acc = fmaf(diff, diff, acc);
+        }
+        if (acc < best) { best = acc; idx = i; }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ob < best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+    }
+    best_sq = best;
+    return idx;
+}
+
+}  // namespace prrtc

@@ -245,11 +245,12 @@ def solve_batch(problems, settings, path_cap: int = 4096):
         sts = (CSettings * n)(*[c_settings(s) for s in settings])
         n_st = n
     res = (CResult * n)()
-    paths = np.zeros((n, path_cap, MAX_DOF), dtype=np.float32)
+    paths = np.zeros((n, path_cap * MAX_DOF), dtype=np.float32)
     stats = CBatchStats()
     rc = load().prrtc_solve_batch(n, cps, sts, n_st, res, paths.ctypes.data, path_cap, C.byref(stats))
     _check(rc, "prrtc_solve_batch")
     results = [_result_dict(res[i]) for i in range(n)]
-    out_paths = [paths[i, :res[i].path_nodes, :dofs[i]].copy() for i in range(n)]
+    # block i holds path_nodes rows of dof floats, packed
+    out_paths = [paths[i, :res[i].path_nodes * dofs[i]].reshape(-1, dofs[i]).copy() for i in range(n)]
     st = {name: getattr(stats, name) for name, _ in CBatchStats._fields_}
     return results, out_paths, st
