@@ -27,7 +27,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-command-line-argument",
     "-munsafe-fp-atomics",
-]
+] + os.environ.get("PRRTC_EXTRA_FLAGS", "").split()
 
 
 def _compile(src: str) -> str:

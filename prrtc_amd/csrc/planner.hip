@@ -348,28 +348,40 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
     CheckCtx ctx;
     ctx.independent = false;
     int staged = -1;
-    int idle = 0;
+    int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
+    const int max_nnc = ctrl->max_nnc;
+    unsigned cursor = blockIdx.x;   // each workgroup walks the problems from its own start
     while (true) {
-        if (load_acquire(&ctrl->finished) >= P) break;
-        unsigned tk = 0;
-        if (lane_id() == 0) tk = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tk = __builtin_amdgcn_readfirstlane(tk);
-        const int p = (int)(tk % (unsigned)P);
+        const int fin = load_acquire(&ctrl->finished);
+        // the running problems can use at most (P - fin) * nnc workgroups: the
+        // rest leave, so a few long problems do not share their counters with
+        // the whole grid
+        if (fin >= P || (long long)blockIdx.x >= (long long)(P - fin) * max_nnc) break;
+        const int p = (int)(cursor % (unsigned)P);
         PlanState *st = states + p;
         const PlanDesc &d = descs[p];
-        // a claim counts only below nnc; finished problems are skipped before
-        // touching their counter, so it never wraps
+        // Only a problem that is running and still has unclaimed slots in this
+        // round is touched with an atomic (plain loads first keep the counters'
+        // cache lines cheap to share).  A claim counts only below nnc, and
+        // finished problems are never incremented, so the counter cannot wrap.
         const bool running = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
-        const int s = running ? wave_atomic_add(&st->next_slot, 1) : d.nnc;
+        const bool open = running && __builtin_amdgcn_readfirstlane(
+                                         __hip_atomic_load(&st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < d.nnc;
+        const int s = open ? wave_atomic_add(&st->next_slot, 1) : d.nnc;
         if (s < 0 || s >= d.nnc) {
+            // move on; after a sweep over every problem without work, sleep
+            // (exponential back-off, up to ~8 x 64 x 64 cycles, a few us)
+            cursor++;
             if (++idle >= P) {
-                __builtin_amdgcn_s_sleep(8);
+                for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
+                backoff = backoff < 8 ? 2 * backoff : 8;
                 idle = 0;
             }
             continue;
         }
         idle = 0;
+        backoff = 1;
         if (staged != p) {
             __syncthreads();
             bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
@@ -616,6 +628,7 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     *hc = PlanControl{};
     hc->n_problems = P;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
+    for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     // --- upload (descs + control in one copy; scenes and init rows per problem)
     hipStream_t s = 0;
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));

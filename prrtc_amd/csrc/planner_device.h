@@ -72,7 +72,8 @@ struct PlanControl {
     int n_problems;
     int path_rows;             // rows used in the path output
     int path_cap;              // rows available in the path output
-    int pad[3];
+    int max_nnc;               // largest num_new_configs of the launch
+    int pad[2];
 };
 
 // slot outcome flags in meta[1]
@@ -172,7 +173,25 @@ __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const 
 #pragma clang fp contract(off)
     float best = __builtin_huge_valf();
     int idx = 0x7fffffff;
-    for (int i = lane_id(); i < n; i += kWave) {
+    int i = lane_id();
+    // four nodes per lane per step keep 4*D independent loads in flight; each
+    // lane still visits its nodes in increasing index order
+    for (; i + 3 * kWave < n; i += 4 * kWave) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const float *col = tree + (size_t)d * cap + i;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float diff = col[u * kWave] - q[d];
+                acc[u] = fmaf(diff, diff, acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (acc[u] < best) { best = acc[u]; idx = i + u * kWave; }
+    }
+    for (; i < n; i += kWave) {
         float acc = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
