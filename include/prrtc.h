@@ -24,6 +24,7 @@ extern "C" {
 #define PRRTC_ENOMEM -3      /* device allocation / admission budget exceeded */
 #define PRRTC_EOVERFLOW -4   /* a caller buffer (path, result) is too small */
 #define PRRTC_EBUSY -5       /* server queue full */
+#define PRRTC_EAGAIN -6      /* server result not ready yet */
 
 /* Robot ids: the reference's explicit instantiations (src/planning/pRRTC.cu:768-771). */
 #define PRRTC_ROBOT_SPHERE 0 /* 3-DoF point robot, radius 1 (src/planning/Robots.hh:190-237) */
@@ -143,6 +144,60 @@ typedef struct prrtc_batch_stats {
 int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
                       int n_settings, prrtc_result *results, float *paths, int path_cap,
                       prrtc_batch_stats *stats);
+
+/* ---- serving (no reference counterpart: the reference serves one blocking
+ *      solve per process at a time, src/planning/pRRTC.cu:607-766) ----
+ * A server owns a device-memory budget and a worker thread.  Requests queue
+ * in arrival order and are admitted while their device footprint fits the
+ * budget; admitted problems run in time-sliced persistent launches (one
+ * stream per robot) and keep their state in device memory between slices.
+ * A request's result (path, iters, tree sizes, cost) is the one prrtc_solve
+ * returns for the same inputs, whatever else is resident.  All server calls
+ * are thread-safe. */
+typedef struct prrtc_server prrtc_server;
+
+typedef struct prrtc_server_config {
+    uint64_t budget_bytes;    /* device memory for resident problems (0: 2 GiB) */
+    int32_t max_resident;     /* resident problems per robot (0: 1024) */
+    int32_t max_pending;      /* queued requests before submit returns EBUSY (0: 65536) */
+    int32_t slice_us;         /* length of one persistent launch (0: 1000) */
+    int32_t path_cap;         /* rows kept per result path (0: 1024) */
+} prrtc_server_config;
+
+typedef struct prrtc_server_stats {
+    int64_t submitted;
+    int64_t completed;        /* results produced (polled or not) */
+    int64_t solved;
+    int32_t pending;          /* queued, not yet admitted */
+    int32_t resident;         /* admitted, not finished */
+    uint64_t resident_bytes;  /* budget in use */
+    uint64_t peak_resident_bytes;
+    int64_t slices;           /* persistent launches */
+    int64_t kernel_ns;        /* device time over all slices */
+    int64_t edge_checks;
+} prrtc_server_stats;
+
+typedef struct prrtc_request_timing {
+    int64_t queued_ns;        /* submit -> admission */
+    int64_t resident_ns;      /* admission -> result */
+    int32_t slices;           /* persistent launches the request ran in */
+} prrtc_request_timing;
+
+void prrtc_server_config_default(prrtc_server_config *cfg);
+int prrtc_server_create(const prrtc_server_config *cfg, prrtc_server **out);
+/* Copies the problem (start, goals, scene) and settings; *ticket names the
+ * request (> 0).  EBUSY: queue full; ENOMEM: footprint above the budget. */
+int prrtc_server_submit(prrtc_server *srv, const prrtc_problem *problem, const prrtc_settings *settings,
+                        uint64_t *ticket);
+/* Waits up to wait_ms (< 0: no limit) for the ticket's result.  PRRTC_OK:
+ * result, path (rows goal -> start, at most path_cap) and timing (may be
+ * NULL) written and the ticket released; EAGAIN: not finished yet. */
+int prrtc_server_poll(prrtc_server *srv, uint64_t ticket, int32_t wait_ms, prrtc_result *result, float *path,
+                      int32_t path_cap, prrtc_request_timing *timing);
+int prrtc_server_get_stats(prrtc_server *srv, prrtc_server_stats *out);
+/* Stops the worker after its current slice and frees the server; queued,
+ * resident and unpolled requests are dropped. */
+int prrtc_server_destroy(prrtc_server *srv);
 
 #ifdef __cplusplus
 }

@@ -5,12 +5,12 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
 
-#include "host_common.h"
-#include "planner_device.h"
+#include "planner_host.h"
 
 namespace prrtc {
 
@@ -350,14 +350,32 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
     int staged = -1;
     int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
+    const int R = ctrl->n_running;
     const int max_nnc = ctrl->max_nnc;
+    const unsigned long long slice = ctrl->slice_ticks;
+    unsigned long long t0 = 0;
+    if (slice) {
+        // one start time for the whole launch: the first workgroup sets it
+        unsigned long long now = __builtin_amdgcn_s_memrealtime();
+        if (lane_id() == 0) {
+            unsigned long long expect = 0;
+            __hip_atomic_compare_exchange_strong(&ctrl->t0, &expect, now, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            t0 = expect ? expect : now;
+        }
+        t0 = ((unsigned long long)__builtin_amdgcn_readfirstlane((int)(t0 >> 32)) << 32) |
+             (unsigned)__builtin_amdgcn_readfirstlane((int)(t0 & 0xffffffffu));
+    }
     unsigned cursor = blockIdx.x;   // each workgroup walks the problems from its own start
     while (true) {
         const int fin = load_acquire(&ctrl->finished);
-        // the running problems can use at most (P - fin) * nnc workgroups: the
+        // the running problems can use at most (R - fin) * nnc workgroups: the
         // rest leave, so a few long problems do not share their counters with
         // the whole grid
-        if (fin >= P || (long long)blockIdx.x >= (long long)(P - fin) * max_nnc) break;
+        if (fin >= R || (long long)blockIdx.x >= (long long)(R - fin) * max_nnc) break;
+        // a time-sliced launch ends between tasks; unfinished rounds resume in
+        // the next launch from the state kept in device memory
+        if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
         const int p = (int)(cursor % (unsigned)P);
         PlanState *st = states + p;
         const PlanDesc &d = descs[p];
@@ -396,10 +414,12 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
 
 // Per-problem setup: tree roots, Halton states (pRRTC.cu:644-693).
 template <class Robot>
-__global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict__ descs, PlanState *states) {
+__global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict__ descs, PlanState *states,
+                                                     const int *__restrict__ list) {
     constexpr int D = Robot::DOF;
-    const PlanDesc &d = descs[blockIdx.x];
-    PlanState *st = states + blockIdx.x;
+    const int e = list ? list[blockIdx.x] : (int)blockIdx.x;   // problem (table entry) to set up
+    const PlanDesc &d = descs[e];
+    PlanState *st = states + e;
     const int lane = lane_id();
     for (int i = lane; i < 1 + d.n_goals; i += kWave) {
         const int t = i == 0 ? 0 : 1, idx = i == 0 ? 0 : i - 1;
@@ -468,16 +488,6 @@ static Workspace &workspace() {
     return *ws;
 }
 
-struct HostProblem {
-    int robot;
-    const float *start;
-    const float *goals;
-    int n_goals;
-    const prrtc_scene *scene;
-    const prrtc_settings *st;
-    int index;                  // position in the caller's arrays
-};
-
 static int settings_error(const prrtc_settings *s) {
     if (!s) return 1;
     if (s->num_new_configs < 1 || s->num_new_configs > kMaxNnc) return 2;
@@ -491,7 +501,7 @@ static int settings_error(const prrtc_settings *s) {
     return 0;
 }
 
-static int robot_limits(int robot, float *lo, float *span) {
+int robot_limits(int robot, float *lo, float *span) {
     switch (robot) {
 #define PRRTC_LIM(ID, R) \
     case ID: for (int i = 0; i < R::DOF; i++) { lo[i] = R::lower(i); span[i] = R::span(i); } return R::DOF;
@@ -506,7 +516,7 @@ static int robot_limits(int robot, float *lo, float *span) {
 
 // Longest possible connect chain: the diagonal of the box spanned by the
 // joint limits, the start and the goals, over `range`.
-static int max_extensions(const HostProblem &hp) {
+int max_extensions(const HostProblem &hp) {
     float lo[kMaxDof], span[kMaxDof];
     const int D = robot_limits(hp.robot, lo, span);
     double diag = 0.0;
@@ -523,19 +533,127 @@ static int max_extensions(const HostProblem &hp) {
     return (int)std::ceil(std::sqrt(diag) / hp.st->range) + 2;
 }
 
-template <class Robot>
-static void launch_plan(const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths, int P,
-                        int grid, int lds_scene_f4, int max_ext, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan_init<Robot>, dim3(P), dim3(kWave), 0, s, d_desc, d_state);
-    PRRTC_HIP(hipGetLastError());
-    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + (size_t)(max_ext + 6) * Robot::DOF * sizeof(float);
-    hipLaunchKernelGGL(k_plan<Robot>, dim3(grid), dim3(kWave), lds, s, d_desc, d_state, d_ctrl, d_paths,
-                       lds_scene_f4, max_ext);
+BlockLayout block_layout(const HostProblem &hp) {
+    const int D = robot_dof(hp.robot);
+    const int nnc = hp.st->num_new_configs;
+    BlockLayout L;
+    L.scene_floats = (int)scene_packed_floats(hp.scene);
+    L.cap = std::max(hp.st->max_samples, hp.n_goals);
+    L.max_ext = max_extensions(hp);
+    Align a;
+    L.scene = a.take(0);
+    a.off += sizeof(float) * (size_t)L.scene_floats;   // init rows follow the scene directly
+    L.init = a.off;
+    a.off += sizeof(float) * (size_t)(1 + hp.n_goals) * D;
+    L.upload_bytes = a.off;
+    a.off = (a.off + 255) & ~size_t(255);
+    L.nodes = a.take(sizeof(float) * 2 * (size_t)D * L.cap);
+    L.parent = a.take(sizeof(int) * 2 * (size_t)L.cap);
+    L.radii = a.take(sizeof(float) * 2 * (size_t)L.cap);
+    L.halton = a.take(sizeof(float) * 3 * (size_t)D * nnc);
+    L.tprev = a.take(sizeof(int) * (size_t)nnc);
+    L.meta = a.take(sizeof(int) * 8 * (size_t)nnc);
+    L.stage = a.take(sizeof(float) * (size_t)nnc * (L.max_ext + 1) * D);
+    L.bytes = a.off;
+    return L;
+}
+
+void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host) {
+    const int D = robot_dof(hp.robot);
+    scene_pack(hp.scene, reinterpret_cast<float *>(host + L.scene));
+    float *ini = reinterpret_cast<float *>(host + L.init);
+    std::memcpy(ini, hp.start, sizeof(float) * D);
+    std::memcpy(ini + D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D);
+}
+
+void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev) {
+    d = PlanDesc{};
+    d.nnc = hp.st->num_new_configs;
+    d.g = hp.st->granularity;
+    d.max_iters = hp.st->max_iters;
+    d.max_samples = hp.st->max_samples;
+    d.balance = hp.st->balance;
+    d.dd = hp.st->dynamic_domain ? 1 : 0;
+    d.n_goals = hp.n_goals;
+    d.cap = L.cap;
+    d.max_ext = L.max_ext;
+    d.scene_f4 = (L.scene_floats - 8) / 4;
+    d.range = hp.st->range;
+    d.tree_ratio = hp.st->tree_ratio;
+    d.dd_alpha = hp.st->dd_alpha;
+    d.dd_radius = hp.st->dd_radius;
+    d.dd_min_radius = hp.st->dd_min_radius;
+    d.scene = reinterpret_cast<const float *>(dev + L.scene);
+    d.nodes = reinterpret_cast<float *>(dev + L.nodes);
+    d.parent = reinterpret_cast<int *>(dev + L.parent);
+    d.radii = reinterpret_cast<float *>(dev + L.radii);
+    d.halton = reinterpret_cast<float *>(dev + L.halton);
+    d.tprev = reinterpret_cast<int *>(dev + L.tprev);
+    d.meta = reinterpret_cast<int *>(dev + L.meta);
+    d.stage = reinterpret_cast<float *>(dev + L.stage);
+    d.init = reinterpret_cast<const float *>(dev + L.init);
+    d.seed = hp.st->seed;
+}
+
+int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int path_cap, float *out) {
+    const int D = robot_dof(hp.robot);
+    const int n_rows = ps.t_len + ps.o_len + 2;
+    if (n_rows > path_cap) return -1;
+    const float *tr = rows + (size_t)ps.path_off * D;      // t-trace then o-trace
+    const float *p1 = ps.t_tree == 1 ? tr : tr + (size_t)ps.t_len * D;
+    const float *p0 = ps.t_tree == 1 ? tr + (size_t)ps.t_len * D : tr;
+    const int n1 = ps.t_tree == 1 ? ps.t_len : ps.o_len;
+    const int n0 = ps.t_tree == 1 ? ps.o_len : ps.t_len;
+    int row = 0;
+    std::memcpy(out + (size_t)row++ * D, hp.goals + (size_t)ps.reached * D, sizeof(float) * D);
+    for (int k = n1 - 1; k >= 0; k--) std::memcpy(out + (size_t)row++ * D, p1 + (size_t)k * D, sizeof(float) * D);
+    for (int k = 0; k < n0; k++) std::memcpy(out + (size_t)row++ * D, p0 + (size_t)k * D, sizeof(float) * D);
+    std::memcpy(out + (size_t)row++ * D, hp.start, sizeof(float) * D);
+    return row;
+}
+
+void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, const int *d_list, int n,
+                      hipStream_t s) {
+    if (n <= 0) return;
+    switch (robot) {
+#define PRRTC_INIT_CASE(ID, R) \
+    case ID: hipLaunchKernelGGL(k_plan_init<R>, dim3(n), dim3(kWave), 0, s, d_desc, d_state, d_list); break;
+        PRRTC_INIT_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
+        PRRTC_INIT_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
+        PRRTC_INIT_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
+        PRRTC_INIT_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+#undef PRRTC_INIT_CASE
+    }
     PRRTC_HIP(hipGetLastError());
 }
 
-// Workgroups per CU of the persistent grid (0 = default).
-static int g_wg_per_cu = 0;
+void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
+                       int grid, int lds_scene_f4, int max_ext, hipStream_t s) {
+    const int D = robot_dof(robot);
+    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + (size_t)(max_ext + 6) * D * sizeof(float);
+    switch (robot) {
+#define PRRTC_SLICE_CASE(ID, R)                                                                              \
+    case ID:                                                                                                 \
+        hipLaunchKernelGGL(k_plan<R>, dim3(grid), dim3(kWave), lds, s, d_desc, d_state, d_ctrl, d_paths,     \
+                           lds_scene_f4, max_ext);                                                           \
+        break;
+        PRRTC_SLICE_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
+        PRRTC_SLICE_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
+        PRRTC_SLICE_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
+        PRRTC_SLICE_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+#undef PRRTC_SLICE_CASE
+    }
+    PRRTC_HIP(hipGetLastError());
+}
+
+int plan_wg_per_cu() {
+    static const int v = [] {
+        const char *e = std::getenv("PRRTC_WG_PER_CU");
+        const int n = e ? std::atoi(e) : 0;
+        return n >= 1 && n <= 32 ? n : 8;
+    }();
+    return v;
+}
 
 // Solve problems that share one robot in one persistent launch.
 static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
@@ -549,99 +667,48 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
     const size_t o_state = dl.take(sizeof(PlanState) * P);
     const size_t o_ctrl = dl.take(sizeof(PlanControl));
-    std::vector<size_t> o_scene(P), o_nodes(P), o_parent(P), o_radii(P), o_halton(P), o_tprev(P), o_meta(P),
-        o_stage(P), o_init(P);
-    std::vector<std::vector<float>> packed(P);
-    std::vector<int> max_ext(P), cap(P);
+    std::vector<BlockLayout> lay(P);
+    std::vector<size_t> o_block(P);
     int lds_f4 = 0, ext_all = 0;
     int64_t path_rows = 0;
     for (int i = 0; i < P; i++) {
-        const HostProblem &hp = probs[i];
-        packed[i].resize(scene_packed_floats(hp.scene));
-        scene_pack(hp.scene, packed[i].data());
-        lds_f4 = std::max(lds_f4, (int)(packed[i].size() - 8) / 4);
-        max_ext[i] = max_extensions(hp);
-        ext_all = std::max(ext_all, max_ext[i]);
-        cap[i] = std::max(hp.st->max_samples, hp.n_goals);
-        const int nnc = hp.st->num_new_configs;
-        o_scene[i] = dl.take(sizeof(float) * packed[i].size());
-        o_nodes[i] = dl.take(sizeof(float) * 2 * (size_t)D * cap[i]);
-        o_parent[i] = dl.take(sizeof(int) * 2 * (size_t)cap[i]);
-        o_radii[i] = dl.take(sizeof(float) * 2 * (size_t)cap[i]);
-        o_halton[i] = dl.take(sizeof(float) * 3 * (size_t)D * nnc);
-        o_tprev[i] = dl.take(sizeof(int) * nnc);
-        o_meta[i] = dl.take(sizeof(int) * 8 * (size_t)nnc);
-        o_stage[i] = dl.take(sizeof(float) * (size_t)nnc * (max_ext[i] + 1) * D);
-        o_init[i] = dl.take(sizeof(float) * (size_t)(1 + hp.n_goals) * D);
+        lay[i] = block_layout(probs[i]);
+        lds_f4 = std::max(lds_f4, (lay[i].scene_floats - 8) / 4);
+        ext_all = std::max(ext_all, lay[i].max_ext);
+        o_block[i] = dl.take(lay[i].bytes);
         path_rows += std::max(0, path_cap - 2);
     }
     const size_t o_paths = dl.take(sizeof(float) * (size_t)path_rows * D);
     char *dev = ws.need_dev(dl.off);
-    // --- host image of everything that is uploaded: descs, control, scenes, init rows
+    // --- host image of everything that is uploaded: descs + control, then
+    // each problem's scene and init rows
     const size_t up_bytes = o_ctrl + sizeof(PlanControl);
     Align hl;
     const size_t h_hdr = hl.take(up_bytes);
-    std::vector<std::pair<size_t, size_t>> extra;   // (host offset, device offset, via size below)
-    std::vector<size_t> h_scene(P), h_init(P);
-    for (int i = 0; i < P; i++) {
-        h_scene[i] = hl.take(sizeof(float) * packed[i].size());
-        h_init[i] = hl.take(sizeof(float) * (size_t)(1 + probs[i].n_goals) * D);
-    }
+    std::vector<size_t> h_block(P);
+    for (int i = 0; i < P; i++) h_block[i] = hl.take(lay[i].upload_bytes);
     const size_t h_read = hl.take(sizeof(PlanState) * P + sizeof(PlanControl));
     char *host = ws.need_host(hl.off);
     PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + h_hdr + o_desc);
     for (int i = 0; i < P; i++) {
-        const HostProblem &hp = probs[i];
-        PlanDesc &d = hd[i];
-        d = PlanDesc{};
-        d.nnc = hp.st->num_new_configs;
-        d.g = hp.st->granularity;
-        d.max_iters = hp.st->max_iters;
-        d.max_samples = hp.st->max_samples;
-        d.balance = hp.st->balance;
-        d.dd = hp.st->dynamic_domain ? 1 : 0;
-        d.n_goals = hp.n_goals;
-        d.cap = cap[i];
-        d.max_ext = max_ext[i];
-        d.scene_f4 = (int)(packed[i].size() - 8) / 4;
-        d.range = hp.st->range;
-        d.tree_ratio = hp.st->tree_ratio;
-        d.dd_alpha = hp.st->dd_alpha;
-        d.dd_radius = hp.st->dd_radius;
-        d.dd_min_radius = hp.st->dd_min_radius;
-        d.scene = reinterpret_cast<const float *>(dev + o_scene[i]);
-        d.nodes = reinterpret_cast<float *>(dev + o_nodes[i]);
-        d.parent = reinterpret_cast<int *>(dev + o_parent[i]);
-        d.radii = reinterpret_cast<float *>(dev + o_radii[i]);
-        d.halton = reinterpret_cast<float *>(dev + o_halton[i]);
-        d.tprev = reinterpret_cast<int *>(dev + o_tprev[i]);
-        d.meta = reinterpret_cast<int *>(dev + o_meta[i]);
-        d.stage = reinterpret_cast<float *>(dev + o_stage[i]);
-        d.init = reinterpret_cast<const float *>(dev + o_init[i]);
-        d.seed = hp.st->seed;
-        std::memcpy(host + h_scene[i], packed[i].data(), sizeof(float) * packed[i].size());
-        float *ini = reinterpret_cast<float *>(host + h_init[i]);
-        std::memcpy(ini, hp.start, sizeof(float) * D);
-        std::memcpy(ini + D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D);
+        fill_desc(hd[i], probs[i], lay[i], dev + o_block[i]);
+        block_upload_image(probs[i], lay[i], host + h_block[i]);
     }
     PlanControl *hc = reinterpret_cast<PlanControl *>(host + h_hdr + o_ctrl);
     *hc = PlanControl{};
     hc->n_problems = P;
+    hc->n_running = P;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     // --- upload (descs + control in one copy; scenes and init rows per problem)
     hipStream_t s = 0;
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
-    for (int i = 0; i < P; i++) {
-        PRRTC_HIP(hipMemcpyAsync(dev + o_scene[i], host + h_scene[i], sizeof(float) * packed[i].size(),
-                                 hipMemcpyHostToDevice, s));
-        PRRTC_HIP(hipMemcpyAsync(dev + o_init[i], host + h_init[i],
-                                 sizeof(float) * (size_t)(1 + probs[i].n_goals) * D, hipMemcpyHostToDevice, s));
-    }
+    for (int i = 0; i < P; i++)
+        PRRTC_HIP(hipMemcpyAsync(dev + o_block[i], host + h_block[i], lay[i].upload_bytes, hipMemcpyHostToDevice, s));
     // --- launch
     int total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
-    const int per_cu = g_wg_per_cu > 0 ? g_wg_per_cu : 8;
+    const int per_cu = plan_wg_per_cu();
     const int grid = std::max(1, std::min(total_slots, num_cus() * per_cu));
     hipEvent_t e0, e1;
     PRRTC_HIP(hipEventCreate(&e0));
@@ -651,12 +718,8 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
     PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
     float *d_paths = reinterpret_cast<float *>(dev + o_paths);
-    switch (robot) {
-        case PRRTC_ROBOT_SPHERE: launch_plan<SphereRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
-        case PRRTC_ROBOT_PANDA: launch_plan<PandaRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
-        case PRRTC_ROBOT_FETCH: launch_plan<FetchRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
-        case PRRTC_ROBOT_BAXTER: launch_plan<BaxterRobot>(d_desc, d_state, d_ctrl, d_paths, P, grid, lds_f4, ext_all, s); break;
-    }
+    launch_plan_init(robot, d_desc, d_state, nullptr, P, s);
+    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid, lds_f4, ext_all, s);
     PRRTC_HIP(hipEventRecord(e1, s));
     // --- read back states + control, then the used path rows
     PlanState *hs = reinterpret_cast<PlanState *>(host + h_read);
@@ -689,29 +752,18 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
         r.path_nodes = 0;
         r.path_length = 0;
         if (!r.solved) continue;
-        const int n_rows = ps.t_len + ps.o_len + 2;
-        if (n_rows > path_cap) {
+        const int row = assemble_path(hp, ps, rows.data(), path_cap, paths + (size_t)hp.index * path_stride);
+        if (row < 0) {
             r.solved = 0;
             r.status = PRRTC_EOVERFLOW;
             continue;
         }
-        float *out = paths + (size_t)hp.index * path_stride;
-        const float *tr = rows.data() + (size_t)ps.path_off * D;      // t-trace then o-trace
-        const float *p1 = ps.t_tree == 1 ? tr : tr + (size_t)ps.t_len * D;
-        const float *p0 = ps.t_tree == 1 ? tr + (size_t)ps.t_len * D : tr;
-        const int n1 = ps.t_tree == 1 ? ps.t_len : ps.o_len;
-        const int n0 = ps.t_tree == 1 ? ps.o_len : ps.t_len;
-        int row = 0;
-        std::memcpy(out + (size_t)row++ * D, hp.goals + (size_t)ps.reached * D, sizeof(float) * D);
-        for (int k = n1 - 1; k >= 0; k--) std::memcpy(out + (size_t)row++ * D, p1 + (size_t)k * D, sizeof(float) * D);
-        for (int k = 0; k < n0; k++) std::memcpy(out + (size_t)row++ * D, p0 + (size_t)k * D, sizeof(float) * D);
-        std::memcpy(out + (size_t)row++ * D, hp.start, sizeof(float) * D);
         r.path_nodes = row;
         r.path_length = ps.t_len + ps.o_len;
     }
 }
 
-static int check_problem(const HostProblem &hp, const char *fn) {
+int check_problem(const HostProblem &hp, const char *fn) {
     const int D = robot_dof(hp.robot);
     if (D < 0) { set_error("%s: unknown robot id %d", fn, hp.robot); return PRRTC_EINVAL; }
     if (!hp.start || !hp.goals || hp.n_goals < 1) { set_error("%s: start/goals missing", fn); return PRRTC_EINVAL; }

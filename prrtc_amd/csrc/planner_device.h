@@ -67,13 +67,16 @@ struct PlanState {
 
 // Launch-wide control block.
 struct PlanControl {
-    unsigned int ticket;       // round-robin problem picker
-    int finished;              // problems no longer running
-    int n_problems;
+    unsigned int ticket;       // unused (kept for layout)
+    int finished;              // problems that stopped running during this launch
+    int n_problems;            // table size (running or not)
     int path_rows;             // rows used in the path output
     int path_cap;              // rows available in the path output
     int max_nnc;               // largest num_new_configs of the launch
-    int pad[2];
+    int n_running;             // problems running when the launch started
+    int pad;
+    unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
+    unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
 };
 
 // slot outcome flags in meta[1]

@@ -1,0 +1,61 @@
+// Host side of the planner shared by the batched solver (planner.hip) and
+// the serving loop (server.hip): request validation, the per-problem device
+// block layout, and launch wrappers that dispatch on the robot id.
+#pragma once
+
+#include "host_common.h"
+#include "planner_device.h"
+
+namespace prrtc {
+
+// One planning problem as handed to the C ABI (pointers stay the caller's).
+struct HostProblem {
+    int robot;
+    const float *start;
+    const float *goals;
+    int n_goals;
+    const prrtc_scene *scene;
+    const prrtc_settings *st;
+    int index;                  // position in the caller's arrays
+};
+
+// Device block of one problem: everything its PlanDesc points to.  The
+// packed scene and the init rows (start, goals) lead the block so one copy
+// uploads both.
+struct BlockLayout {
+    size_t scene = 0, init = 0, nodes = 0, parent = 0, radii = 0, halton = 0, tprev = 0, meta = 0, stage = 0;
+    size_t upload_bytes = 0;    // scene + init, contiguous at the block start
+    size_t bytes = 0;           // whole block (256-byte multiple)
+    int cap = 0, max_ext = 0, scene_floats = 0;
+};
+
+// Rejects what the planner cannot run (unknown robot, NaN, oversized scene,
+// bad settings, chains over kMaxExt); sets the thread's error message.
+int check_problem(const HostProblem &hp, const char *fn);
+int robot_limits(int robot, float *lo, float *span);
+int max_extensions(const HostProblem &hp);
+BlockLayout block_layout(const HostProblem &hp);
+// Writes the packed scene and init rows to `host` (layout.upload_bytes).
+void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host);
+// Fills a PlanDesc whose buffers live in the device block at `dev`.
+void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev);
+
+// Single-wave workgroups per CU of a persistent planner grid (environment
+// variable PRRTC_WG_PER_CU, 1..32; default 8).
+int plan_wg_per_cu();
+
+// Tree roots and Halton states of the problems listed in d_list (table
+// entries), or of entries 0..n-1 when d_list is null.
+void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, const int *d_list, int n,
+                      hipStream_t s);
+// One persistent launch over the problem table (runs until every running
+// problem stops, or until the control block's time slice ends).
+void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
+                       int grid, int lds_scene_f4, int max_ext, hipStream_t s);
+
+// Rows of one solution: goal -> start (pRRTC.cu:715-746) from the two traces
+// the commit wrote at state.path_off.  Returns the row count, or -1 if the
+// path does not fit path_cap rows.
+int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int path_cap, float *out);
+
+}  // namespace prrtc

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libprrtc.so")
 
 PRRTC_OK = 0
-ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUSY"}
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUSY", -6: "EAGAIN"}
+PRRTC_EAGAIN = -6
 MAX_DOF = 14
 
 _f32p = C.POINTER(C.c_float)
@@ -64,6 +65,25 @@ class CBatchStats(C.Structure):
     ]
 
 
+class CServerConfig(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64), ("max_resident", C.c_int32), ("max_pending", C.c_int32),
+        ("slice_us", C.c_int32), ("path_cap", C.c_int32),
+    ]
+
+
+class CServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("solved", C.c_int64), ("pending", C.c_int32),
+        ("resident", C.c_int32), ("resident_bytes", C.c_uint64), ("peak_resident_bytes", C.c_uint64),
+        ("slices", C.c_int64), ("kernel_ns", C.c_int64), ("edge_checks", C.c_int64),
+    ]
+
+
+class CRequestTiming(C.Structure):
+    _fields_ = [("queued_ns", C.c_int64), ("resident_ns", C.c_int64), ("slices", C.c_int32)]
+
+
 # (name, restype, argtypes) of every symbol include/prrtc.h declares
 SIGNATURES = [
     ("prrtc_settings_default", None, [C.POINTER(CSettings)]),
@@ -82,6 +102,14 @@ SIGNATURES = [
                               C.POINTER(CResult), C.c_void_p, C.c_int]),
     ("prrtc_solve_batch", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
                                     C.POINTER(CResult), C.c_void_p, C.c_int, C.POINTER(CBatchStats)]),
+    ("prrtc_server_config_default", None, [C.POINTER(CServerConfig)]),
+    ("prrtc_server_create", C.c_int, [C.POINTER(CServerConfig), C.POINTER(C.c_void_p)]),
+    ("prrtc_server_submit", C.c_int, [C.c_void_p, C.POINTER(CProblem), C.POINTER(CSettings),
+                                      C.POINTER(C.c_uint64)]),
+    ("prrtc_server_poll", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(CResult), C.c_void_p, C.c_int32,
+                                    C.POINTER(CRequestTiming)]),
+    ("prrtc_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(CServerStats)]),
+    ("prrtc_server_destroy", C.c_int, [C.c_void_p]),
 ]
 
 _lib = None
@@ -254,3 +282,79 @@ def solve_batch(problems, settings, path_cap: int = 4096):
     out_paths = [paths[i, :res[i].path_nodes * dofs[i]].reshape(-1, dofs[i]).copy() for i in range(n)]
     st = {name: getattr(stats, name) for name, _ in CBatchStats._fields_}
     return results, out_paths, st
+
+
+def _c_problem(p, keep):
+    rid = robot_id(p["robot"])
+    D = load().prrtc_robot_dof(rid)
+    s = _f(p["start"]).reshape(D)
+    g = _f(p["goals"]).reshape(-1, D)
+    cs = c_scene(p["scene"])
+    keep += [s, g, cs]
+    cp = CProblem()
+    cp.robot = rid
+    cp.start = s.ctypes.data_as(_f32p)
+    cp.goals = g.ctypes.data_as(_f32p)
+    cp.n_goals = len(g)
+    cp.scene = cs
+    return cp, D
+
+
+class Server:
+    """Continuous-batching server (prrtc_server_* in include/prrtc.h).
+
+    submit() copies the problem and returns a ticket; poll(ticket) returns
+    (result, path, timing) once the problem has stopped, else None."""
+
+    def __init__(self, budget_bytes=0, max_resident=0, max_pending=0, slice_us=0, path_cap=0):
+        cfg = CServerConfig(budget_bytes, max_resident, max_pending, slice_us, path_cap)
+        h = C.c_void_p()
+        _check(load().prrtc_server_create(C.byref(cfg), C.byref(h)), "prrtc_server_create")
+        self._h = h
+        self._dof = {}
+        self.path_cap = path_cap or 1024
+
+    def submit(self, problem: dict, settings: dict | None = None) -> int:
+        keep = []
+        cp, D = _c_problem(problem, keep)
+        st = c_settings(settings or {})
+        t = C.c_uint64()
+        _check(load().prrtc_server_submit(self._h, C.byref(cp), C.byref(st), C.byref(t)), "prrtc_server_submit")
+        self._dof[t.value] = D
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = -1):
+        D = self._dof[ticket]
+        res = CResult()
+        tm = CRequestTiming()
+        path = np.zeros((self.path_cap, D), dtype=np.float32)
+        rc = load().prrtc_server_poll(self._h, ticket, wait_ms, C.byref(res), path.ctypes.data, self.path_cap,
+                                      C.byref(tm))
+        if rc == PRRTC_EAGAIN:
+            return None
+        _check(rc, "prrtc_server_poll")
+        del self._dof[ticket]
+        timing = {name: getattr(tm, name) for name, _ in CRequestTiming._fields_}
+        return _result_dict(res), path[:res.path_nodes].copy(), timing
+
+    def stats(self) -> dict:
+        st = CServerStats()
+        _check(load().prrtc_server_get_stats(self._h, C.byref(st)), "prrtc_server_get_stats")
+        return {name: getattr(st, name) for name, _ in CServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            load().prrtc_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
