@@ -145,6 +145,19 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
                       int n_settings, prrtc_result *results, float *paths, int path_cap,
                       prrtc_batch_stats *stats);
 
+/* Launch shape of the persistent planner per robot (no reference
+ * counterpart: the reference launches num_new_configs blocks of 128 threads
+ * per problem, src/planning/pRRTC.cu:700-702).  wg_per_cu: single-wave
+ * workgroups per CU of the grid, 1..32, 0 = default (environment variable
+ * PRRTC_WG_PER_CU, else 8).  robot = -1 sets every robot.  Applies to later
+ * prrtc_solve, prrtc_solve_batch and server slices. */
+typedef struct prrtc_tuning {
+    int32_t wg_per_cu;
+} prrtc_tuning;
+
+int prrtc_set_tuning(int robot, const prrtc_tuning *t);
+int prrtc_get_tuning(int robot, prrtc_tuning *t);
+
 /* ---- serving (no reference counterpart: the reference serves one blocking
  *      solve per process at a time, src/planning/pRRTC.cu:607-766) ----
  * A server owns a device-memory budget and a worker thread.  Requests queue

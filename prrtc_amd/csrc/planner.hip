@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cstdlib>
 #include <map>
@@ -11,6 +12,12 @@
 #include <vector>
 
 #include "planner_host.h"
+
+// Minimum waves per SIMD the persistent planner is compiled for (bounds its
+// registers: 1 -> up to 512, 2 -> 256).
+#ifndef PRRTC_PLAN_WAVES_PER_EU
+#define PRRTC_PLAN_WAVES_PER_EU 1
+#endif
 
 namespace prrtc {
 
@@ -208,7 +215,7 @@ __device__ __forceinline__ float dd_apply(const PlanDesc &d, float r, bool grow)
 // Commit a finished round (run by the wave that finished its last task).
 // Returns the problem's new status.
 template <class Robot>
-__device__ int plan_commit(const PlanDesc &d, PlanState *st, PlanControl *ctrl, float *paths) {
+__device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl *ctrl, float *paths) {
     constexpr int D = Robot::DOF;
     const int lane = lane_id();
     int sz[2] = {st->size[0], st->size[1]};
@@ -333,6 +340,8 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, PlanControl *ctrl, 
             store_release(&st->next_slot, 0);
         } else {
             store_release(&st->next_slot, kClaimedOut);
+            __hip_atomic_fetch_and(ctrl->run_mask + (p >> 6), ~(1ull << (p & 63)), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             atomic_add_agent(&ctrl->finished, 1);
         }
     }
@@ -340,7 +349,8 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, PlanControl *ctrl, 
 }
 
 template <class Robot>
-__global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PRRTC_PLAN_WAVES_PER_EU)))
+void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
                                                 PlanControl *ctrl, float *paths, int lds_scene_f4, int lds_ext) {
     extern __shared__ float4 lds[];
     float *chain = reinterpret_cast<float *>(lds + lds_scene_f4);
@@ -366,7 +376,8 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
         t0 = ((unsigned long long)__builtin_amdgcn_readfirstlane((int)(t0 >> 32)) << 32) |
              (unsigned)__builtin_amdgcn_readfirstlane((int)(t0 & 0xffffffffu));
     }
-    unsigned cursor = blockIdx.x;   // each workgroup walks the problems from its own start
+    const int words = ctrl->mask_words;
+    int cursor = (int)(blockIdx.x % (unsigned)P);   // each workgroup starts at its own problem
     while (true) {
         const int fin = load_acquire(&ctrl->finished);
         // the running problems can use at most (R - fin) * nnc workgroups: the
@@ -376,28 +387,36 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
         // a time-sliced launch ends between tasks; unfinished rounds resume in
         // the next launch from the state kept in device memory
         if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
-        const int p = (int)(cursor % (unsigned)P);
-        PlanState *st = states + p;
-        const PlanDesc &d = descs[p];
-        // Only a problem that is running and still has unclaimed slots in this
-        // round is touched with an atomic (plain loads first keep the counters'
-        // cache lines cheap to share).  A claim counts only below nnc, and
-        // finished problems are never incremented, so the counter cannot wrap.
-        const bool running = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
-        const bool open = running && __builtin_amdgcn_readfirstlane(
-                                         __hip_atomic_load(&st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < d.nnc;
-        const int s = open ? wave_atomic_add(&st->next_slot, 1) : d.nnc;
-        if (s < 0 || s >= d.nnc) {
-            // move on; after a sweep over every problem without work, sleep
-            // (exponential back-off, up to ~8 x 64 x 64 cycles, a few us)
-            cursor++;
-            if (++idle >= P) {
+        // the next running problem from the cursor on (finished ones are
+        // skipped through the mask, not visited one by one)
+        const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor));
+        int s = -1, nnc = 0;
+        PlanState *st = nullptr;
+        if (p >= 0) {
+            st = states + p;
+            nnc = descs[p].nnc;
+            // Only a problem that is running and still has unclaimed slots in
+            // this round is touched with an atomic (plain loads first keep the
+            // counters' cache lines cheap to share).  A claim counts only below
+            // nnc, and finished problems are never incremented, so the counter
+            // cannot wrap.
+            const bool running = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
+            const bool open = running && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                             &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
+            if (open) s = wave_atomic_add(&st->next_slot, 1);
+        }
+        if (s < 0 || s >= nnc) {
+            // move on; after visiting every running problem without work,
+            // sleep (exponential back-off, up to 4 x 64 x 64 cycles)
+            if (p >= 0) cursor = p + 1 < P ? p + 1 : 0;
+            if (p < 0 || ++idle >= R - fin) {
                 for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
-                backoff = backoff < 8 ? 2 * backoff : 8;
+                backoff = backoff < 4 ? 2 * backoff : 4;
                 idle = 0;
             }
             continue;
         }
+        const PlanDesc &d = descs[p];
         idle = 0;
         backoff = 1;
         if (staged != p) {
@@ -408,7 +427,7 @@ __global__ __launch_bounds__(kWave) void k_plan(const PlanDesc *__restrict__ des
         plan_task<Robot>(d, st, s, ctx, chain, uv);
         __threadfence();
         const int prev = wave_atomic_add(&st->done_slots, 1);
-        if (prev == d.nnc - 1) plan_commit<Robot>(d, st, ctrl, paths);
+        if (prev == d.nnc - 1) plan_commit<Robot>(d, st, p, ctrl, paths);
     }
 }
 
@@ -538,7 +557,9 @@ BlockLayout block_layout(const HostProblem &hp) {
     const int nnc = hp.st->num_new_configs;
     BlockLayout L;
     L.scene_floats = (int)scene_packed_floats(hp.scene);
-    L.cap = std::max(hp.st->max_samples, hp.n_goals);
+    // storage rounded up to 64 nodes (16-byte aligned SoA columns); the
+    // node limit stays max_samples
+    L.cap = (std::max(hp.st->max_samples, hp.n_goals) + 63) & ~63;
     L.max_ext = max_extensions(hp);
     Align a;
     L.scene = a.take(0);
@@ -646,13 +667,21 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
     PRRTC_HIP(hipGetLastError());
 }
 
-int plan_wg_per_cu() {
+// Workgroups per CU of each robot's persistent grid (prrtc_set_tuning).
+static std::atomic<int> g_wg_per_cu[4] = {{0}, {0}, {0}, {0}};
+
+static int default_wg_per_cu() {
     static const int v = [] {
         const char *e = std::getenv("PRRTC_WG_PER_CU");
         const int n = e ? std::atoi(e) : 0;
-        return n >= 1 && n <= 32 ? n : 8;
+        return n >= 1 && n <= kMaxWgPerCu ? n : 8;
     }();
     return v;
+}
+
+int plan_wg_per_cu(int robot) {
+    const int v = robot >= 0 && robot < 4 ? g_wg_per_cu[robot].load(std::memory_order_relaxed) : 0;
+    return v > 0 ? v : default_wg_per_cu();
 }
 
 // Solve problems that share one robot in one persistent launch.
@@ -667,6 +696,8 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
     const size_t o_state = dl.take(sizeof(PlanState) * P);
     const size_t o_ctrl = dl.take(sizeof(PlanControl));
+    const int words = (P + 63) / 64;
+    const size_t o_mask = dl.take(sizeof(unsigned long long) * words);
     std::vector<BlockLayout> lay(P);
     std::vector<size_t> o_block(P);
     int lds_f4 = 0, ext_all = 0;
@@ -682,7 +713,7 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     char *dev = ws.need_dev(dl.off);
     // --- host image of everything that is uploaded: descs + control, then
     // each problem's scene and init rows
-    const size_t up_bytes = o_ctrl + sizeof(PlanControl);
+    const size_t up_bytes = o_mask + sizeof(unsigned long long) * words;
     Align hl;
     const size_t h_hdr = hl.take(up_bytes);
     std::vector<size_t> h_block(P);
@@ -698,6 +729,10 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     *hc = PlanControl{};
     hc->n_problems = P;
     hc->n_running = P;
+    hc->mask_words = words;
+    hc->run_mask = reinterpret_cast<unsigned long long *>(dev + o_mask);
+    unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + h_hdr + o_mask);
+    for (int w = 0; w < words; w++) hm[w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     // --- upload (descs + control in one copy; scenes and init rows per problem)
@@ -708,7 +743,7 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     // --- launch
     int total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
-    const int per_cu = plan_wg_per_cu();
+    const int per_cu = plan_wg_per_cu(robot);
     const int grid = std::max(1, std::min(total_slots, num_cus() * per_cu));
     hipEvent_t e0, e1;
     PRRTC_HIP(hipEventCreate(&e0));
@@ -835,6 +870,25 @@ int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, 
         std::vector<HostProblem> probs{HostProblem{robot, start, goals, n_goals, scene, settings, 0}};
         return solve_many(probs, result, path, path_cap, 0, nullptr, "prrtc_solve");
     });
+}
+
+int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
+    if (!t || robot < -1 || robot > 3 || t->wg_per_cu < 0 || t->wg_per_cu > kMaxWgPerCu) {
+        set_error("prrtc_set_tuning: invalid argument");
+        return PRRTC_EINVAL;
+    }
+    for (int r = 0; r < 4; r++)
+        if (robot < 0 || r == robot) g_wg_per_cu[r].store(t->wg_per_cu, std::memory_order_relaxed);
+    return PRRTC_OK;
+}
+
+int prrtc_get_tuning(int robot, prrtc_tuning *t) {
+    if (!t || robot < 0 || robot > 3) {
+        set_error("prrtc_get_tuning: invalid argument");
+        return PRRTC_EINVAL;
+    }
+    t->wg_per_cu = plan_wg_per_cu(robot);
+    return PRRTC_OK;
 }
 
 int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,

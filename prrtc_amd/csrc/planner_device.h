@@ -67,16 +67,17 @@ struct PlanState {
 
 // Launch-wide control block.
 struct PlanControl {
-    unsigned int ticket;       // unused (kept for layout)
     int finished;              // problems that stopped running during this launch
     int n_problems;            // table size (running or not)
     int path_rows;             // rows used in the path output
     int path_cap;              // rows available in the path output
     int max_nnc;               // largest num_new_configs of the launch
     int n_running;             // problems running when the launch started
+    int mask_words;            // ceil(n_problems / 64)
     int pad;
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
+    unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit
 };
 
 // slot outcome flags in meta[1]
@@ -168,33 +169,78 @@ __host__ __device__ inline void halton_bases(uint64_t seed, int slot, int dof, f
     for (int i = 0; i < dof; i++) out[i] = primes[i];
 }
 
+// First problem at or after `from` (cyclically) whose bit is set in the
+// running mask, or -1 if none: one load per lane covers 64 mask words.
+__device__ __forceinline__ int next_running(const unsigned long long *mask, int words, int from) {
+    const int lane = lane_id();
+    const int w0 = from >> 6;
+    const unsigned long long hi_part = ~0ull << (from & 63), lo_part = ~hi_part;
+    int first_a = -1, first_b = -1;   // a: at or after `from`; b: before it (wrap-around)
+    for (int base = 0; base < words; base += kWave) {
+        const int w = base + lane;
+        const unsigned long long m =
+            w < words ? __hip_atomic_load(mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const unsigned long long a = w > w0 ? m : (w == w0 ? m & hi_part : 0ull);
+        const unsigned long long b = w < w0 ? m : (w == w0 ? m & lo_part : 0ull);
+        const uint64_t ba = ballot(a != 0ull), bb = ballot(b != 0ull);
+        if (first_b < 0 && bb) {
+            const int j = __builtin_ctzll(bb);
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, j);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), j);
+            first_b = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
+        }
+        if (ba) {
+            const int j = __builtin_ctzll(ba);
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)a, j);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(a >> 32), j);
+            first_a = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
+            break;
+        }
+    }
+    return first_a >= 0 ? first_a : first_b;
+}
+
 // Nearest node of tree t to q among [0, n): squared L2 as an fmaf chain in
 // dimension order (utils.cuh:100-109), ties to the smaller index.  The wave
 // strides the SoA coordinates (coalesced) and reduces by lane shuffles.
+// `tree` and `cap` keep every column 16-byte aligned (cap % 4 == 0).
 template <int D>
 __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq) {
 #pragma clang fp contract(off)
     float best = __builtin_huge_valf();
     int idx = 0x7fffffff;
-    int i = lane_id();
-    // four nodes per lane per step keep 4*D independent loads in flight; each
-    // lane still visits its nodes in increasing index order
-    for (; i + 3 * kWave < n; i += 4 * kWave) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int lane = lane_id();
+    // 1024-node chunks: lane l reads nodes c + 256u + 4l .. +3 (u = 0..3) of
+    // every column as float4, so 4*D 16-byte loads are in flight per lane;
+    // each lane still visits its nodes in increasing index order
+    int c = 0;
+    for (; c + 16 * kWave <= n; c += 16 * kWave) {
+        float acc[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            const float *col = tree + (size_t)d * cap + i;
+            const float4 *col = reinterpret_cast<const float4 *>(tree + (size_t)d * cap + c) + lane;
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = col[u * kWave];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const float diff = col[u * kWave] - q[d];
-                acc[u] = fmaf(diff, diff, acc[u]);
+                float diff = v[u].x - q[d];
+                acc[4 * u + 0] = fmaf(diff, diff, acc[4 * u + 0]);
+                diff = v[u].y - q[d];
+                acc[4 * u + 1] = fmaf(diff, diff, acc[4 * u + 1]);
+                diff = v[u].z - q[d];
+                acc[4 * u + 2] = fmaf(diff, diff, acc[4 * u + 2]);
+                diff = v[u].w - q[d];
+                acc[4 * u + 3] = fmaf(diff, diff, acc[4 * u + 3]);
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (acc[u] < best) { best = acc[u]; idx = i + u * kWave; }
+        for (int e = 0; e < 16; e++)
+            if (acc[e] < best) { best = acc[e]; idx = c + (e >> 2) * 4 * kWave + 4 * lane + (e & 3); }
     }
-    for (; i < n; i += kWave) {
+    for (int i = c + lane; i < n; i += kWave) {
         float acc = 0.0f;
 #pragma unroll
         for (int d = 0; d < D; d++) {

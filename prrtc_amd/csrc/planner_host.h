@@ -40,9 +40,11 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
 // Fills a PlanDesc whose buffers live in the device block at `dev`.
 void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev);
 
-// Single-wave workgroups per CU of a persistent planner grid (environment
-// variable PRRTC_WG_PER_CU, 1..32; default 8).
-int plan_wg_per_cu();
+// Single-wave workgroups per CU of a robot's persistent planner grid: the
+// value set by prrtc_set_tuning, else the environment variable
+// PRRTC_WG_PER_CU (1..32), else 8.
+constexpr int kMaxWgPerCu = 32;
+int plan_wg_per_cu(int robot);
 
 // Tree roots and Halton states of the problems listed in d_list (table
 // entries), or of entries 0..n-1 when d_list is null.

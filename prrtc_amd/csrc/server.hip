@@ -115,6 +115,7 @@ struct RobotClass {
     PlanState *d_state = nullptr, *h_state = nullptr;
     PlanControl *d_ctrl = nullptr, *h_ctrl_up = nullptr, *h_ctrl_down = nullptr;
     int *d_list = nullptr, *h_list = nullptr;
+    unsigned long long *d_mask = nullptr, *h_mask = nullptr;   // running entries
     float *d_paths = nullptr;
     int path_rows = 0;
     char *h_stage = nullptr;
@@ -137,6 +138,7 @@ struct RobotClass {
         PRRTC_HIP(hipMalloc(&d_state, sizeof(PlanState) * E));
         PRRTC_HIP(hipMalloc(&d_ctrl, sizeof(PlanControl)));
         PRRTC_HIP(hipMalloc(&d_list, sizeof(int) * E));
+        PRRTC_HIP(hipMalloc(&d_mask, sizeof(unsigned long long) * ((E + 63) / 64)));
         path_rows = E * std::max(1, path_cap - 2);
         PRRTC_HIP(hipMalloc(&d_paths, sizeof(float) * (size_t)path_rows * D));
         // status 0 marks a free entry the planner skips
@@ -146,6 +148,7 @@ struct RobotClass {
         h_ctrl_up = pinned<PlanControl>(1);
         h_ctrl_down = pinned<PlanControl>(1);
         h_list = pinned<int>(E);
+        h_mask = pinned<unsigned long long>((E + 63) / 64);
         entries.assign(E, nullptr);
         for (int e = 0; e < E; e++) free_entries.insert(e);
         PRRTC_HIP(hipStreamSynchronize(stream));
@@ -158,12 +161,14 @@ struct RobotClass {
         (void)hipFree(d_state);
         (void)hipFree(d_ctrl);
         (void)hipFree(d_list);
+        (void)hipFree(d_mask);
         (void)hipFree(d_paths);
         (void)hipHostFree(h_desc);
         (void)hipHostFree(h_state);
         (void)hipHostFree(h_ctrl_up);
         (void)hipHostFree(h_ctrl_down);
         (void)hipHostFree(h_list);
+        (void)hipHostFree(h_mask);
         if (h_stage) (void)hipHostFree(h_stage);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -272,9 +277,12 @@ void ServerImpl::launch(RobotClass &c) {
     }
     int lds_f4 = 0, ext = 0, max_nnc = 0;
     long long slots = 0;
+    const int words = (c.high + 63) / 64;
+    for (int w = 0; w < words; w++) c.h_mask[w] = 0ull;
     for (int e = 0; e < c.high; e++) {
         const Request *rq = c.entries[e];
         if (!rq) continue;
+        c.h_mask[e >> 6] |= 1ull << (e & 63);   // every resident entry is running
         lds_f4 = std::max(lds_f4, (rq->lay.scene_floats - 8) / 4);
         ext = std::max(ext, rq->lay.max_ext);
         max_nnc = std::max(max_nnc, rq->st.num_new_configs);
@@ -287,8 +295,11 @@ void ServerImpl::launch(RobotClass &c) {
     hc.path_cap = c.path_rows;
     hc.max_nnc = max_nnc;
     hc.slice_ticks = (unsigned long long)cfg.slice_us * ticks_per_us;
+    hc.mask_words = words;
+    hc.run_mask = c.d_mask;
+    PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * words, hipMemcpyHostToDevice, s));
     PRRTC_HIP(hipMemcpyAsync(c.d_ctrl, c.h_ctrl_up, sizeof(PlanControl), hipMemcpyHostToDevice, s));
-    const int grid = (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * plan_wg_per_cu()));
+    const int grid = (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * plan_wg_per_cu(c.robot)));
     PRRTC_HIP(hipEventRecord(c.e0, s));
     launch_plan_slice(c.robot, c.d_desc, c.d_state, c.d_ctrl, c.d_paths, grid, lds_f4, ext, s);
     PRRTC_HIP(hipEventRecord(c.e1, s));

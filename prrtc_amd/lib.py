@@ -65,6 +65,10 @@ class CBatchStats(C.Structure):
     ]
 
 
+class CTuning(C.Structure):
+    _fields_ = [("wg_per_cu", C.c_int32)]
+
+
 class CServerConfig(C.Structure):
     _fields_ = [
         ("budget_bytes", C.c_uint64), ("max_resident", C.c_int32), ("max_pending", C.c_int32),
@@ -102,6 +106,8 @@ SIGNATURES = [
                               C.POINTER(CResult), C.c_void_p, C.c_int]),
     ("prrtc_solve_batch", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
                                     C.POINTER(CResult), C.c_void_p, C.c_int, C.POINTER(CBatchStats)]),
+    ("prrtc_set_tuning", C.c_int, [C.c_int, C.POINTER(CTuning)]),
+    ("prrtc_get_tuning", C.c_int, [C.c_int, C.POINTER(CTuning)]),
     ("prrtc_server_config_default", None, [C.POINTER(CServerConfig)]),
     ("prrtc_server_create", C.c_int, [C.POINTER(CServerConfig), C.POINTER(C.c_void_p)]),
     ("prrtc_server_submit", C.c_int, [C.c_void_p, C.POINTER(CProblem), C.POINTER(CSettings),
@@ -176,6 +182,19 @@ def settings_default() -> dict:
     st = CSettings()
     load().prrtc_settings_default(C.byref(st))
     return {name: getattr(st, name) for name, _ in CSettings._fields_}
+
+
+def set_tuning(robot, wg_per_cu: int) -> None:
+    """Workgroups per CU of the persistent planner grid for `robot` (or every
+    robot when robot is None); 0 restores the default."""
+    t = CTuning(int(wg_per_cu))
+    _check(load().prrtc_set_tuning(-1 if robot is None else robot_id(robot), C.byref(t)), "prrtc_set_tuning")
+
+
+def get_tuning(robot) -> dict:
+    t = CTuning()
+    _check(load().prrtc_get_tuning(robot_id(robot), C.byref(t)), "prrtc_get_tuning")
+    return {"wg_per_cu": t.wg_per_cu}
 
 
 def pack_scene(scene: Scene) -> np.ndarray:
