@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the pRRTC hot path on one MI355X.
 
-Workload (one step = one pass of the hot path over one batch): the Panda
-edge-validation microbench -- 1M random edges (endpoints uniform in the joint
-limits, length <= 0.5 rad) at granularity 128 against 24 oriented cuboids
-and 8 vertical cylinders.  Inputs are resident in HBM before timing; each
-step is one launch of the edge-validation kernel over all edges.
+Headline workload (one step = one pass of the hot path over one batch): the
+Panda edge-validation microbench -- 1M random edges (endpoints uniform in the
+joint limits, length <= 0.5 rad) at granularity 128 against 24 oriented
+cuboids and 8 vertical cylinders.  Inputs are resident in HBM before timing;
+each step is one launch of the edge-validation kernel over all edges.
 
-    value        edge checks / s of the timed steps (edges whose verdict was
-                 decided, all of them)
+    value        edge checks / s of the timed steps (all edges decided)
     roofline     the validation kernel on the collision-free subset of the
                  same edges (every pose fully evaluated): reference flops /
-                 kernel time, against MI355X's 157.3 TFLOP/s FP32 vector peak
-    plan         the configuration's single-problem plan (g=128, dynamic
-                 domain) over seeded start/goal pairs: p50/p99 solve ms
-    batched      plans/s of a batch of Fetch problems in one persistent launch
-                 vs the same problems one launch each (identical results)
-    cpu_baseline the CPU oracle on a bounded sample of the same edges,
-                 all host threads; paths re-validated on the CPU
+                 kernel time against MI355X's FP32 vector peak
+    plan         single-problem plans of the same scene (g=128, dynamic
+                 domain): p50/p99 solve ms
+    c0           plain Panda problem (12 spheres, nnc 128, g 32) over seeds:
+                 p50/p99 solve ms, success, cost; CPU RRT-Connect beside it
+    batched      256 Fetch problems (8-64 mixed obstacles, g 64): plans/s of
+                 one persistent launch vs one launch per problem
+    mix          128 Baxter + 256 Panda + 128 Fetch shelf problems (g 128,
+                 dynamic domain, balance 1) in one call
+    serving      Poisson request stream (Panda:Fetch:Baxter 2:1:1, fresh
+                 4-128 primitive scenes) through the continuous-batching
+                 server: p50/p99 latency, admission wait, plans/s
+    cpu_baseline the C oracle on a bounded sample of the same edges on the
+                 host's cores; every returned path re-validated on the CPU
 
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
@@ -34,6 +40,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_PACKED = 157.3e12
 PEAK_FP32 = 78.6e12
+HBM_PEAK = 8.0e12
 
 
 def log(msg):
@@ -53,6 +60,19 @@ def time_launches(fn, steps, torch):
     return e0.elapsed_time(e1) / steps
 
 
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+
+
+def load_tuned():
+    """Launch shapes and num_new_configs picked by prrtc_amd.autotune."""
+    path = os.path.join(ROOT, "prrtc_amd", "tuned.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,11 +81,18 @@ def main():
     ap.add_argument("--edges", type=int, default=1_000_000)
     ap.add_argument("--granularity", type=int, default=128)
     ap.add_argument("--plans", type=int, default=16, help="single-problem plans for p50/p99")
+    ap.add_argument("--c0-seeds", type=int, default=32)
     ap.add_argument("--batch", type=int, default=256, help="Fetch problems of the batched vs plain comparison")
-    ap.add_argument("--nnc", type=int, default=64, help="num_new_configs of the batched problems")
-    ap.add_argument("--cpu-edges", type=int, default=4000)
+    ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
+    ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--cpu-edges", type=int, default=500_000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plans", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only the free-set validation launches and one 256 MiB device copy (for rocprofv3 "
+                         "counter runs: the copy's counters scale FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: independent problems shard across GPUs with no exchange, "
@@ -82,6 +109,9 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream().cuda_stream
     g = args.granularity
+    tuned = load_tuned()
+    for robot, t in tuned.get("launch", {}).items():
+        P.set_tuning(robot, t.get("wg_per_cu", 0), t.get("waves_per_task", 0))
 
     # ---------------------------------------------------------------- inputs
     log("building the scene and the edges")
@@ -96,11 +126,14 @@ def main():
         P.validate_edges_async("panda", d_scene.data_ptr(), packed.size, d_edges.data_ptr(), args.edges, g,
                                d_out.data_ptr(), stream)
 
-    for _ in range(args.warmup):
+    for _ in range(0 if args.profile_run else args.warmup):
         step()
     torch.cuda.synchronize()
-    log(f"timing {args.steps} steps of {args.edges} edges")
-    ms = time_launches(step, args.steps, torch)
+    main_steps = 1 if args.profile_run else args.steps   # a profile run needs only the free set
+    log(f"timing {main_steps} steps of {args.edges} edges")
+    t_wall = time.perf_counter()
+    ms = time_launches(step, main_steps, torch)
+    t_wall = time.perf_counter() - t_wall
     free = d_out.cpu().numpy().astype(bool)
     value = args.edges / (ms / 1e3)
     log(f"{ms:.3f} ms/step, {value / 1e6:.2f} M edge checks/s, {free.mean() * 100:.1f}% of edges free")
@@ -121,7 +154,15 @@ def main():
     roof_step()
     roof_ms = time_launches(roof_step, args.steps, torch)
     assert bool(d_roof_out.all()), "free-edge set must validate free"
-    # reference flops per pose, from a sample of the free poses
+    if args.profile_run:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)       # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_run": True, "validate_launches": args.steps + 1, "edges_per_launch": n_roof,
+                          "kernel_ms": roof_ms, "copy_bytes_read": src.numel() * 4,
+                          "copy_bytes_written": dst.numel() * 4}), flush=True)
+        return
     rng = np.random.default_rng(5)
     pick = roof_edges[rng.integers(0, n_roof, 512)]
     ks = rng.integers(1, g + 1, size=(512, 1)).astype(np.float32)
@@ -131,10 +172,14 @@ def main():
     flops = per_pose * n_roof * g
     achieved = flops / (roof_ms / 1e3)
     log(f"free set: {n_roof} edges, {roof_ms:.3f} ms, {per_pose:.0f} flops/pose, {achieved / 1e12:.2f} TFLOP/s")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "validate_edges_panda_free_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": "edge checks/sec (Panda, 1M random edges, g=128); plans/sec batched vs one-per-launch; "
-                  "p50/p99 solve ms",
+        "metric": "plans/sec batched vs one-per-launch; p50/p99 solve ms; edge checks/sec",
         "value": value,
         "unit": "edge checks/s",
         "n_gpus": 1,
@@ -145,22 +190,23 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded scenes and edges)",
+        "data": "synthetic (seeded scenes, edges and problems)",
         "config": {"workload": "panda edge-validation microbench: 1M random edges <= 0.5 rad, granularity 128, "
-                               "24 oriented cuboids + 8 vertical cylinders",
+                               "24 oriented cuboids + 8 vertical cylinders (capsules)",
                    "edges": args.edges, "granularity": g, "free_edge_fraction": float(free.mean()),
                    "parallelism": "single GPU"},
+        "timed_wall_s": t_wall,
         "pose_checks_per_s_free": n_roof * g / (roof_ms / 1e3),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_PACKED / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_PACKED,
-                     "frac_of_nonpacked_peak": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "k_validate_edges<PandaRobot>", "kernel_ms": roof_ms,
+                     "frac_of_nonpacked_peak": achieved / PEAK_FP32, "traffic": traffic,
+                     "kernel": "k_validate_edges<PandaRobot, true>", "kernel_ms": roof_ms,
                      "flops_per_pose": per_pose, "poses_per_launch": n_roof * g},
     }
+    paths_to_check = []   # (robot, scene, path, g)
 
-    # ----------------------------------------------------------------- plans
-    plan_paths = []
     if not args.skip_plans:
+        # ------------------------------------------------ plan (same scene)
         log(f"single-problem plans: {args.plans}")
         probs = workloads.c1_problems(args.plans, seed=1)
         st = dict(num_new_configs=128, granularity=128, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=1,
@@ -173,69 +219,174 @@ def main():
             times.append((time.perf_counter() - t0) * 1e3)
             solved += res["solved"]
             if res["solved"]:
-                plan_paths.append(("panda", p["scene"], path, 128))
-        out["plan"] = {"problems": len(probs), "solved": solved, "p50_ms": float(np.percentile(times, 50)),
-                       "p99_ms": float(np.percentile(times, 99)), "settings": "nnc 128, g 128, dd on (r 4, "
-                       "alpha 1e-4, min 1), range 0.5, balance 2"}
+                paths_to_check.append(("panda", p["scene"], path, 128))
+        out["plan"] = {"problems": len(probs), "solved": solved, "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
+                       "settings": "panda, nnc 128, g 128, dd on (r 4, alpha 1e-4, min 1), range 0.5, balance 2"}
         log(f"plans: {solved}/{len(probs)} solved, p50 {out['plan']['p50_ms']:.1f} ms, "
             f"p99 {out['plan']['p99_ms']:.1f} ms")
 
-        # batched vs plain
+        # ------------------------------------------------------------- c0
+        c0 = workloads.c0_problem(0)
+        c0_st = dict(num_new_configs=128, granularity=32, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0,
+                     max_samples=100000, max_iters=1000000)
+        times, costs, solved = [], [], 0
+        for seed in range(args.c0_seeds):
+            t0 = time.perf_counter()
+            res, path = P.solve("panda", c0["start"], c0["goals"], c0["scene"], dict(c0_st, seed=seed))
+            times.append((time.perf_counter() - t0) * 1e3)
+            solved += res["solved"]
+            if res["solved"]:
+                costs.append(res["cost"])
+                paths_to_check.append(("panda", c0["scene"], path, 32))
+        out["c0"] = {"seeds": args.c0_seeds, "solved": solved, "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
+                     "cost_p50": pct(costs, 50), "cost_p90": pct(costs, 90),
+                     "settings": "panda, 12 spheres, nnc 128, g 32, range 0.5, balance 2, ratio 1, dd off"}
+        log(f"c0: {solved}/{args.c0_seeds} solved, p50 {out['c0']['p50_ms']:.2f} ms, p99 {out['c0']['p99_ms']:.2f} ms")
+
+        # -------------------------------------------------- batched vs plain
         log(f"batched vs plain: {args.batch} Fetch problems")
         bprobs = workloads.c2_problems(args.batch, seed=0)
-        bst = dict(num_new_configs=args.nnc, granularity=64, range=0.5, balance=2, tree_ratio=1.0,
-                   dynamic_domain=0, max_samples=100000, max_iters=2000, seed=1)
-        P.solve_batch(bprobs[:8], bst)                                                       # warm
+        base = dict(granularity=64, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0, max_samples=100000,
+                    seed=1)
+        nnc_b = tuned.get("batch_nnc", {}).get("fetch", 128)
+        nnc_p = tuned.get("plain_nnc", {}).get("fetch", 256)
+        P.solve_batch(bprobs[:8], dict(base, num_new_configs=nnc_b, max_iters=50))            # warm
+        # problems the planner connects within 3000 rounds at nnc 64 (a fixed
+        # property of each problem: start and goal in one free-space component)
+        pres, _, _ = P.solve_batch(bprobs, dict(base, num_new_configs=64, max_iters=3000))
+        feas = [p for p, r in zip(bprobs, pres) if r["solved"]]
+        bst = dict(base, num_new_configs=nnc_b, max_iters=20000)
+        pst = dict(base, num_new_configs=nnc_p, max_iters=20000)
         t0 = time.perf_counter()
-        bres, bpaths, bstats = P.solve_batch(bprobs, bst)
+        bres, bpaths, bstats = P.solve_batch(feas, bst)
         t_batch = time.perf_counter() - t0
         t0 = time.perf_counter()
-        plain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], bst) for p in bprobs]
+        plain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], pst) for p in feas]
         t_plain = time.perf_counter() - t0
-        same = all(r["iters"] == pr[0]["iters"] and r["solved"] == pr[0]["solved"] and
-                   pa.tobytes() == pr[1].tobytes() for r, pa, pr in zip(bres, bpaths, plain))
-        n_solved = sum(r["solved"] for r in bres)
-        out["batched"] = {"problems": len(bprobs), "solved": n_solved,
-                          "plans_per_s_batched": n_solved / t_batch,
-                          "plans_per_s_plain": sum(r[0]["solved"] for r in plain) / t_plain,
-                          "speedup": (n_solved / t_batch) / max(1e-9, sum(r[0]["solved"] for r in plain) / t_plain),
+        sub = min(32, len(feas))
+        same = all(r["iters"] == pr[0]["iters"] and pa.tobytes() == pr[1].tobytes() for r, pa, pr in
+                   zip(bres[:sub], bpaths[:sub], [P.solve(p["robot"], p["start"], p["goals"], p["scene"], bst)
+                                                  for p in feas[:sub]]))
+        n_b = sum(r["solved"] for r in bres)
+        n_p = sum(r[0]["solved"] for r in plain)
+        out["batched"] = {"problems": len(bprobs), "feasible": len(feas), "solved_batched": n_b, "solved_plain": n_p,
+                          "plans_per_s_batched": n_b / t_batch, "plans_per_s_plain": n_p / t_plain,
+                          "speedup": (n_b / t_batch) / max(1e-9, n_p / t_plain),
                           "batch_wall_s": t_batch, "plain_wall_s": t_plain,
-                          "batch_kernel_ms": bstats["kernel_ns"] / 1e6,
+                          "batch_kernel_ms": bstats["kernel_ns"] / 1e6, "rounds_max": bstats["rounds"],
                           "edge_checks_per_s_batched": bstats["edge_checks"] / (bstats["kernel_ns"] / 1e9),
-                          "results_identical": bool(same),
-                          "settings": f"fetch, g 64, nnc {args.nnc}, range 0.5, balance 2"}
+                          "nnc_batched": nnc_b, "nnc_plain": nnc_p,
+                          "same_settings_identical_results": bool(same),
+                          "settings": "fetch, g 64, range 0.5, balance 2, dd off; nnc per mode from the autotuner"}
         log(f"batched {out['batched']['plans_per_s_batched']:.1f} plans/s vs plain "
-            f"{out['batched']['plans_per_s_plain']:.1f} plans/s, identical={same}")
-        for p, r, pa in zip(bprobs, bres, bpaths):
+            f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x), identical={same}")
+        for p, r, pa in zip(feas, bres, bpaths):
             if r["solved"]:
-                plan_paths.append(("fetch", p["scene"], pa, 64))
+                paths_to_check.append(("fetch", p["scene"], pa, 64))
+
+        # -------------------------------------------------------------- mix
+        nb, npd, nf = (max(1, int(round(k * args.mix_scale))) for k in (128, 256, 128))
+        log(f"uneven mix: {nb} Baxter + {npd} Panda + {nf} Fetch")
+        mprobs = workloads.c3_problems(nb, npd, nf, seed=0)
+        mnnc = tuned.get("batch_nnc", {})
+        mst = [dict(num_new_configs=mnnc.get(p["robot"], 128), granularity=128, range=0.5, balance=1, tree_ratio=0.5,
+                    dynamic_domain=1, dd_radius=4.0, dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000,
+                    max_iters=3000, seed=1) for p in mprobs]
+        t0 = time.perf_counter()
+        mres, mpaths, mstats = P.solve_batch(mprobs, mst)
+        t_mix = time.perf_counter() - t0
+        n_m = sum(r["solved"] for r in mres)
+        per_robot = {}
+        for p, r in zip(mprobs, mres):
+            d = per_robot.setdefault(p["robot"], {"problems": 0, "solved": 0})
+            d["problems"] += 1
+            d["solved"] += r["solved"]
+        out["mix"] = {"problems": len(mprobs), "solved": n_m, "plans_per_s": n_m / t_mix, "wall_s": t_mix,
+                      "kernel_ms": mstats["kernel_ns"] / 1e6, "per_robot": per_robot,
+                      "settings": "g 128, balance 1, ratio 0.5, dd on, range 0.5, max_iters 3000; one persistent "
+                                  "grid per robot class, all classes concurrently"}
+        log(f"mix: {n_m}/{len(mprobs)} solved in {t_mix:.3f} s ({n_m / t_mix:.1f} plans/s)")
+        for p, r, pa in zip(mprobs, mres, mpaths):
+            if r["solved"]:
+                paths_to_check.append((p["robot"], p["scene"], pa, 128))
+
+    if not args.skip_serving:
+        # ---------------------------------------------------------- serving
+        n_req = max(1, int(args.serve_rate * args.serve_seconds))
+        log(f"serving: {n_req} requests at {args.serve_rate:.0f}/s")
+        reqs = [workloads.serving_request(i, seed=0) for i in range(n_req)]
+        snnc = tuned.get("serve_nnc", {})
+        sst = {r: dict(num_new_configs=snnc.get(r, 64), granularity=64, range=0.5, balance=2, tree_ratio=1.0,
+                       dynamic_domain=0, max_samples=50000, max_iters=3000, seed=1)
+               for r in ("panda", "fetch", "baxter")}
+        arrivals = np.cumsum(np.random.default_rng(11).exponential(1.0 / args.serve_rate, n_req))
+        with P.Server(budget_bytes=16 << 30, slice_us=1000) as srv:
+            t_start = time.perf_counter()
+            tickets = []
+            for a, rq in zip(arrivals, reqs):
+                delay = t_start + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit(rq, sst[rq["robot"]]))
+            outs = [srv.poll(t) for t in tickets]
+            t_end = time.perf_counter()
+            sstats = srv.stats()
+        lat = [(tm["queued_ns"] + tm["resident_ns"]) / 1e6 for _, _, tm in outs]
+        queued = [tm["queued_ns"] / 1e6 for _, _, tm in outs]
+        lat_solved = [l for l, (r, _, _) in zip(lat, outs) if r["solved"]]
+        n_s = sum(r["solved"] for r, _, _ in outs)
+        out["serving"] = {"requests": n_req, "rate_per_s": args.serve_rate, "solved": n_s,
+                          "plans_per_s": n_s / (t_end - t_start), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                          "p50_ms_solved": pct(lat_solved, 50), "p99_ms_solved": pct(lat_solved, 99),
+                          "admission_wait_p50_ms": pct(queued, 50), "admission_wait_p99_ms": pct(queued, 99),
+                          "slices": sstats["slices"], "peak_resident_mib": sstats["peak_resident_bytes"] / 2**20,
+                          "settings": "g 64, nnc 64, range 0.5, balance 2, max_iters 3000; 1 ms slices, 16 GiB budget"}
+        log(f"serving: {n_s}/{n_req} solved, p50 {out['serving']['p50_ms']:.1f} ms, "
+            f"p99 {out['serving']['p99_ms']:.1f} ms, {out['serving']['plans_per_s']:.1f} plans/s")
+        for rq, (r, pa, _) in zip(reqs[:200], outs[:200]):
+            if r["solved"]:
+                paths_to_check.append((rq["robot"], rq["scene"], pa, 64))
 
     # ---------------------------------------------------------- cpu baseline
     if not args.skip_cpu:
         from oracle import oracle
         from prrtc_amd.robots import load_model
         threads = min(16, os.cpu_count() or 1)
-        R = oracle.OracleRobot(load_model("panda"))
-        sample = np.ascontiguousarray(edges[: args.cpu_edges])
-        t0 = time.perf_counter()
-        cpu_free = R.validate_edges(scene, sample, g, threads=threads)
-        dt = time.perf_counter() - t0
-        agree = float((cpu_free == free[: args.cpu_edges]).mean())
-        out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "edge checks/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"first {len(sample)} of the same edges, g={g}, CPU oracle "
-                                         f"(C, -O3), {threads} threads",
-                               "verdict_agreement": agree}
-        log(f"cpu: {len(sample) / dt:.0f} edge checks/s on {threads} threads, agreement {agree:.4f}")
-        # every returned path re-validated edge by edge on the CPU
         models = {}
-        bad = 0
-        for robot, sc, path, gg in plan_paths:
+
+        def model(robot):
             if robot not in models:
                 models[robot] = oracle.OracleRobot(load_model(robot))
-            bad += models[robot].revalidate(sc, path, gg) != -1
-        out["paths_revalidated"] = {"paths": len(plan_paths), "valid": len(plan_paths) - bad}
-        log(f"paths re-validated on the CPU: {len(plan_paths) - bad}/{len(plan_paths)} valid")
+            return models[robot]
+
+        sample = np.ascontiguousarray(edges[: args.cpu_edges])
+        t0 = time.perf_counter()
+        cpu_free = model("panda").validate_edges(scene, sample, g, threads=threads)
+        dt = time.perf_counter() - t0
+        agree = float((cpu_free == free[: args.cpu_edges]).mean())
+        out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "edge checks/s", "cores": threads, "kind": "port",
+                               "sample": f"first {len(sample)} of the same 1M edges, g={g}, C oracle (-O3), "
+                                         f"{threads} threads, static partition",
+                               "seconds": dt, "verdict_agreement": agree}
+        log(f"cpu: {len(sample) / dt:.0f} edge checks/s on {threads} threads, agreement {agree:.4f}")
+        if "c0" in out:
+            c0 = workloads.c0_problem(0)
+            times, solved = [], 0
+            for seed in range(8):
+                t0 = time.perf_counter()
+                rc, res, path = model("panda").rrt_connect(c0["scene"], c0["start"], c0["goals"][0], 32, 0.5,
+                                                           100000, seed)
+                times.append((time.perf_counter() - t0) * 1e3)
+                solved += res.solved
+            out["cpu_rrt_connect_c0"] = {"seeds": 8, "solved": solved, "p50_ms": pct(times, 50),
+                                         "p99_ms": pct(times, 99), "cores": 1,
+                                         "kind": "project CPU RRT-Connect (vamp unavailable offline)"}
+            log(f"cpu rrt-connect c0: p50 {out['cpu_rrt_connect_c0']['p50_ms']:.1f} ms")
+        bad = 0
+        for robot, sc, path, gg in paths_to_check:
+            bad += model(robot).revalidate(sc, path, gg) != -1
+        out["paths_revalidated"] = {"paths": len(paths_to_check), "valid": len(paths_to_check) - bad}
+        log(f"paths re-validated on the CPU: {len(paths_to_check) - bad}/{len(paths_to_check)} valid")
 
     print(json.dumps(out), flush=True)
 

@@ -147,12 +147,17 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
 
 /* Launch shape of the persistent planner per robot (no reference
  * counterpart: the reference launches num_new_configs blocks of 128 threads
- * per problem, src/planning/pRRTC.cu:700-702).  wg_per_cu: single-wave
- * workgroups per CU of the grid, 1..32, 0 = default (environment variable
- * PRRTC_WG_PER_CU, else 8).  robot = -1 sets every robot.  Applies to later
- * prrtc_solve, prrtc_solve_batch and server slices. */
+ * per problem, src/planning/pRRTC.cu:700-702).  0 = default for each field.
+ *   waves_per_task  waves (64 lanes each) that share one planner task: its
+ *                   nearest-node scans and its edge / connect-chain poses,
+ *                   1..4 (environment PRRTC_TASK_WAVES, else 2)
+ *   wg_per_cu       task workgroups per CU of the persistent grid, 1..32
+ *                   (environment PRRTC_WG_PER_CU, else 8 / waves_per_task)
+ * robot = -1 sets every robot.  Applies to later prrtc_solve,
+ * prrtc_solve_batch and server slices; results do not depend on it. */
 typedef struct prrtc_tuning {
     int32_t wg_per_cu;
+    int32_t waves_per_task;
 } prrtc_tuning;
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t);

@@ -18,6 +18,7 @@
 namespace prrtc {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dispatched to them round-robin
 constexpr int kMaxDof = 14;
 // Largest obstacle table set staged into LDS per workgroup.
 constexpr size_t kMaxLdsScene = 64 * 1024;

@@ -23,18 +23,36 @@ namespace prrtc {
 
 // ------------------------------------------------------------------ device
 
-// One task: slot s of problem p in the current round (pRRTC.cu:342-551).
+// Combine one value per wave across the workgroup: the OR (want_min false)
+// or the minimum of `v`.  `buf` holds 2 * kMaxTaskWaves ints in LDS, used by
+// alternate calls (`parity`), so one barrier per call suffices.
+__device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int parity) {
+    const int W = blockDim.x / kWave;
+    if (W == 1) return v;
+    int *b = buf + parity * kMaxTaskWaves;
+    if (lane_id() == 0) b[wave_id()] = v;
+    __syncthreads();
+    int r = b[0];
+    for (int w = 1; w < W; w++) r = want_min ? min(r, b[w]) : (r | b[w]);
+    return r;
+}
+
+// One task: slot s of problem p in the current round (pRRTC.cu:342-551),
+// run by every thread of the workgroup (1..kMaxTaskWaves waves).
 //
 // The steered edge and the connect chain are validated by one loop with a
 // single call site of the robot's check sequence (the sequence is large; one
-// inlined copy keeps register use down).  Uniform vectors live in LDS:
-// chain[i] holds the chain configurations c_0 = new node, c_{i+1} = c_i + vec;
-// uv holds the nearest node, the steered step and the connect step.
+// inlined copy keeps register use down); its poses are spread over all
+// threads.  Uniform vectors live in LDS: chain[i] holds the chain
+// configurations c_0 = new node, c_{i+1} = c_i + vec; uv holds the nearest
+// node, the steered step and the connect step; red and wbuf are the
+// cross-wave reduction buffers.
 template <class Robot>
-__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx, float *chain, float *uv) {
+__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx, float *chain, float *uv,
+                          float *red, int *wbuf) {
 #pragma clang fp contract(off)
     constexpr int D = Robot::DOF;
-    const int lane = lane_id();
+    const int tid = threadIdx.x, NT = blockDim.x;
     const int iter = st->round;
     const int n0 = st->size[0], n1 = st->size[1];
     const int g = d.g;
@@ -44,45 +62,53 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
     float *nnv = uv, *delta = uv + D, *step = uv + 2 * D, *vecv = uv + 3 * D;
     int edges = 0;
 
-    // tree choice and Halton sample (uniform across the wave)
+    // tree choice and Halton sample (uniform across the workgroup)
     const int t = choose_tree(d, s, iter, d.tprev[s], n0, n1);
     const int o = 1 - t;
     float *hs = d.halton + (size_t)s * 3 * D;
-    float sample[D];
+    float sample[D], h_n[D], h_d[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
         float b = hs[k], n = hs[D + k], dd = hs[2 * D + k];
         const float u = halton_step(b, n, dd);
-        if (lane == 0) { hs[D + k] = n; hs[2 * D + k] = dd; }
+        h_n[k] = n;
+        h_d[k] = dd;
         sample[k] = fmaf(u, Robot::span(k), Robot::lower(k));     // Robots.hh scale_cfg
     }
-    if (lane == 0) d.tprev[s] = t;
+    // every wave has read the slot's Halton state and tree before thread 0
+    // advances them
+    if (NT > kWave) __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < D; k++) { hs[D + k] = h_n[k]; hs[2 * D + k] = h_d[k]; }
+        d.tprev[s] = t;
+    }
 
     // nearest node in the target tree (pRRTC.cu:384-418)
     const float *tn = d.nodes + (size_t)t * D * d.cap;
     float bsq;
-    const int nn = nearest<D>(tn, d.cap, t == 0 ? n0 : n1, sample, bsq);
+    const int nn = nearest<D>(tn, d.cap, t == 0 ? n0 : n1, sample, bsq, red);
     const float dist = sqrtf(bsq);
     const float scale = fminf(1.0f, d.range / dist);
     int flags = 0, n_ok = 0, nn_o = 0;
     if (d.dd && d.radii[(size_t)t * d.cap + nn] < dist) {          // pRRTC.cu:420-427
         flags = kSkip;
     } else {
-        if (lane < D) {                                               // pRRTC.cu:430-434
-            const float a = tn[(size_t)lane * d.cap + nn];
+        if (tid < D) {                                                // pRRTC.cu:430-434
+            const float a = tn[(size_t)tid * d.cap + nn];
             float sl = sample[0];
 #pragma unroll
-            for (int k = 1; k < D; k++) if (lane == k) sl = sample[k];
+            for (int k = 1; k < D; k++) if (tid == k) sl = sample[k];
             const float c = fmaf(sl - a, scale, a);
-            nnv[lane] = a;
-            chain[lane] = c;
-            delta[lane] = (c - a) / gf;
+            nnv[tid] = a;
+            chain[tid] = c;
+            delta[tid] = (c - a) / gf;
         }
         __syncthreads();
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
-        const int passes = (g + kWave - 1) / kWave;
-        int phase = 0, p = 0, base = 0, total = 0, n_chain = 0;
+        const int passes = (g + NT - 1) / NT;
+        int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0;
         unsigned n_ext = 0;
         edges = 1;
         while (true) {
@@ -90,13 +116,13 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
             int key = 0;
             bool dead;
             if (phase == 0) {
-                const int k = lane * passes + p;
+                const int k = tid * passes + p;
                 const float kk = (float)((k < g ? k : g - 1) + 1);
 #pragma unroll
                 for (int c = 0; c < D; c++) q[c] = fmaf(kk, delta[c], nnv[c]);
                 dead = k >= g;
             } else {
-                const int j = base + lane;
+                const int j = base + tid;
                 const int jj = j < total ? j : total - 1;
                 const int i = jj / g, k = jj % g;
 #pragma unroll
@@ -110,15 +136,18 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
             Robot::check(q, ctx);
             const uint64_t cm = ballot(ctx.coll);
             if (phase == 1) {
-                if (cm) {                                             // first colliding sub-edge
-                    n_ok = __builtin_amdgcn_readlane(key, __builtin_ctzll(cm));
+                // first colliding sub-edge: waves hold increasing pose ranges
+                const int wfirst = cm ? __builtin_amdgcn_readlane(key, __builtin_ctzll(cm)) : 0x7fffffff;
+                const int first = across_waves(wfirst, true, wbuf, npass++ & 1);
+                if (first != 0x7fffffff) {
+                    n_ok = first;
                     break;
                 }
-                base += kWave;
+                base += NT;
                 if (base >= total) { n_ok = n_chain; break; }
                 continue;
             }
-            if (cm) break;                                            // steered edge collides
+            if (across_waves(cm ? 1 : 0, false, wbuf, npass++ & 1)) break;   // steered edge collides
             if (++p < passes) continue;
             // steered edge free: connect to the other tree (pRRTC.cu:478-518)
             flags |= kExtOk;
@@ -126,19 +155,19 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
 #pragma unroll
             for (int c = 0; c < D; c++) cfg[c] = chain[c];
             const float *on = d.nodes + (size_t)o * D * d.cap;
-            nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq);
+            nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq, red);
             const float dist_o = sqrtf(bsq);
             n_ext = (unsigned)ceilf(dist_o / d.range);
             n_chain = (int)min(n_ext, (unsigned)d.max_ext);
-            if (lane < D) {
-                const float vec = (on[(size_t)lane * d.cap + nn_o] - chain[lane]) / (float)n_ext;
-                vecv[lane] = vec;
-                step[lane] = vec / gf;
+            if (tid < D) {
+                const float vec = (on[(size_t)tid * d.cap + nn_o] - chain[tid]) / (float)n_ext;
+                vecv[tid] = vec;
+                step[tid] = vec / gf;
             }
             __syncthreads();
-            // chain configurations by sequential adds, lane i computes c_i
-            for (int i0 = 1; i0 <= n_chain; i0 += kWave) {
-                const int i = i0 + lane;
+            // chain configurations by sequential adds, thread i computes c_i
+            for (int i0 = 1; i0 <= n_chain; i0 += NT) {
+                const int i = i0 + tid;
                 if (i <= n_chain) {
 #pragma unroll
                     for (int c = 0; c < D; c++) {
@@ -159,8 +188,8 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
             edges += n_ok < n_chain ? n_ok + 1 : n_ok;
             if (n_ok == (int)n_ext) flags |= kConnected;
             // stage the new node and the extension nodes
-            for (int i0 = 0; i0 <= n_ok; i0 += kWave) {
-                const int i = i0 + lane;
+            for (int i0 = 0; i0 <= n_ok; i0 += NT) {
+                const int i = i0 + tid;
                 if (i <= n_ok) {
 #pragma unroll
                     for (int c = 0; c < D; c++) stage[(size_t)i * D + c] = chain[i * D + c];
@@ -169,7 +198,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
         }
         __syncthreads();
     }
-    if (lane == 0) {
+    if (tid == 0) {
         meta[0] = t;
         meta[1] = flags;
         meta[2] = nn;
@@ -349,14 +378,18 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
 }
 
 template <class Robot>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PRRTC_PLAN_WAVES_PER_EU)))
-void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
-                                                PlanControl *ctrl, float *paths, int lds_scene_f4, int lds_ext) {
+__global__ __launch_bounds__(kMaxTaskWaves * kWave) __attribute__((amdgpu_waves_per_eu(PRRTC_PLAN_WAVES_PER_EU)))
+void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *ctrl, float *paths, int lds_scene_f4,
+            int lds_ext) {
     extern __shared__ float4 lds[];
     float *chain = reinterpret_cast<float *>(lds + lds_scene_f4);
     float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
+    float *red = uv + 4 * Robot::DOF;                                 // 2 * kMaxTaskWaves
+    int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);     // 2 * kMaxTaskWaves
+    int *ctl = wbuf + 2 * kMaxTaskWaves;                              // the leader's decision: p, s
     CheckCtx ctx;
     ctx.independent = false;
+    const bool leader = wave_id() == 0;   // wave 0 claims the tasks and commits the rounds
     int staged = -1;
     int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
@@ -364,7 +397,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
     const int max_nnc = ctrl->max_nnc;
     const unsigned long long slice = ctrl->slice_ticks;
     unsigned long long t0 = 0;
-    if (slice) {
+    if (slice && leader) {
         // one start time for the whole launch: the first workgroup sets it
         unsigned long long now = __builtin_amdgcn_s_memrealtime();
         if (lane_id() == 0) {
@@ -379,55 +412,72 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states,
     const int words = ctrl->mask_words;
     int cursor = (int)(blockIdx.x % (unsigned)P);   // each workgroup starts at its own problem
     while (true) {
-        const int fin = load_acquire(&ctrl->finished);
-        // the running problems can use at most (R - fin) * nnc workgroups: the
-        // rest leave, so a few long problems do not share their counters with
-        // the whole grid
-        if (fin >= R || (long long)blockIdx.x >= (long long)(R - fin) * max_nnc) break;
-        // a time-sliced launch ends between tasks; unfinished rounds resume in
-        // the next launch from the state kept in device memory
-        if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
-        // the next running problem from the cursor on (finished ones are
-        // skipped through the mask, not visited one by one)
-        const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor));
-        int s = -1, nnc = 0;
-        PlanState *st = nullptr;
-        if (p >= 0) {
-            st = states + p;
-            nnc = descs[p].nnc;
-            // Only a problem that is running and still has unclaimed slots in
-            // this round is touched with an atomic (plain loads first keep the
-            // counters' cache lines cheap to share).  A claim counts only below
-            // nnc, and finished problems are never incremented, so the counter
-            // cannot wrap.
-            const bool running = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
-            const bool open = running && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                                             &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
-            if (open) s = wave_atomic_add(&st->next_slot, 1);
-        }
-        if (s < 0 || s >= nnc) {
-            // move on; after visiting every running problem without work,
-            // sleep (exponential back-off, up to 4 x 64 x 64 cycles)
-            if (p >= 0) cursor = p + 1 < P ? p + 1 : 0;
-            if (p < 0 || ++idle >= R - fin) {
-                for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
-                backoff = backoff < 4 ? 2 * backoff : 4;
-                idle = 0;
+        if (leader) {
+            int p_out = -1, s_out = -1;   // p_out < 0: the workgroup leaves
+            while (true) {
+                const int fin = load_acquire(&ctrl->finished);
+                // the running problems can use at most (R - fin) * nnc
+                // workgroups: the rest leave, so a few long problems do not
+                // share their counters with the whole grid
+                if (fin >= R || (long long)blockIdx.x >= (long long)(R - fin) * max_nnc) break;
+                // a time-sliced launch ends between tasks; unfinished rounds
+                // resume in the next launch from the state kept in device memory
+                if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
+                // the next running problem from the cursor on (finished ones
+                // are skipped through the mask, not visited one by one)
+                const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor));
+                int s = -1, nnc = 0;
+                if (p >= 0) {
+                    PlanState *st = states + p;
+                    nnc = descs[p].nnc;
+                    // Only a problem that is running and still has unclaimed
+                    // slots in this round is touched with an atomic (plain
+                    // loads first keep the counters' cache lines cheap to
+                    // share).  A claim counts only below nnc, and finished
+                    // problems are never incremented, so the counter cannot wrap.
+                    const bool running =
+                        __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
+                    const bool open = running && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                                     &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
+                    if (open) s = wave_atomic_add(&st->next_slot, 1);
+                }
+                if (s >= 0 && s < nnc) {
+                    p_out = p;
+                    s_out = s;
+                    idle = 0;
+                    backoff = 1;
+                    break;
+                }
+                // move on; after visiting every running problem without work,
+                // sleep (exponential back-off, up to 4 x 64 x 64 cycles)
+                if (p >= 0) cursor = p + 1 < P ? p + 1 : 0;
+                if (p < 0 || ++idle >= R - fin) {
+                    for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
+                    backoff = backoff < 4 ? 2 * backoff : 4;
+                    idle = 0;
+                }
             }
-            continue;
+            if (lane_id() == 0) {
+                ctl[0] = p_out;
+                ctl[1] = s_out;
+            }
         }
+        __syncthreads();
+        const int p = ctl[0], s = ctl[1];
+        if (p < 0) break;
         const PlanDesc &d = descs[p];
-        idle = 0;
-        backoff = 1;
-        if (staged != p) {
-            __syncthreads();
+        PlanState *st = states + p;
+        if (staged != p) {   // every wave is past its last use of the staged scene
             bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
             staged = p;
         }
-        plan_task<Robot>(d, st, s, ctx, chain, uv);
+        plan_task<Robot>(d, st, s, ctx, chain, uv, red, wbuf);
         __threadfence();
-        const int prev = wave_atomic_add(&st->done_slots, 1);
-        if (prev == d.nnc - 1) plan_commit<Robot>(d, st, p, ctrl, paths);
+        __syncthreads();   // the whole task is written, and ctl read, before the leader goes on
+        if (leader) {
+            const int prev = wave_atomic_add(&st->done_slots, 1);
+            if (prev == d.nnc - 1) plan_commit<Robot>(d, st, p, ctrl, paths);
+        }
     }
 }
 
@@ -474,12 +524,21 @@ struct Align {
     }
 };
 
-// Device memory reused across calls of this thread (grown on demand).
+// Device memory, pinned staging, stream and timing events of one robot's
+// group, reused across calls of this thread (grown on demand).
 struct Workspace {
     char *dev = nullptr;
     size_t bytes = 0;
     char *host = nullptr;       // pinned staging for uploads / readback
     size_t host_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void ensure_stream() {
+        if (stream) return;
+        PRRTC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        PRRTC_HIP(hipEventCreate(&e0));
+        PRRTC_HIP(hipEventCreate(&e1));
+    }
     char *need_dev(size_t n) {
         if (n > bytes) {
             if (dev) (void)hipFree(dev);
@@ -502,9 +561,11 @@ struct Workspace {
     }
 };
 
-static Workspace &workspace() {
-    static thread_local Workspace *ws = new Workspace();   // released with the process
-    return *ws;
+static Workspace &workspace(int robot) {
+    static thread_local Workspace *ws[4] = {};   // released with the process
+    if (!ws[robot]) ws[robot] = new Workspace();
+    ws[robot]->ensure_stream();
+    return *ws[robot];
 }
 
 static int settings_error(const prrtc_settings *s) {
@@ -557,9 +618,10 @@ BlockLayout block_layout(const HostProblem &hp) {
     const int nnc = hp.st->num_new_configs;
     BlockLayout L;
     L.scene_floats = (int)scene_packed_floats(hp.scene);
-    // storage rounded up to 64 nodes (16-byte aligned SoA columns); the
-    // node limit stays max_samples
-    L.cap = (std::max(hp.st->max_samples, hp.n_goals) + 63) & ~63;
+    // storage rounded up to whole 1024-node scan chunks (16-byte aligned SoA
+    // columns, the nearest-node scan reads whole chunks); the node limit
+    // stays max_samples
+    L.cap = (std::max(hp.st->max_samples, hp.n_goals) + kScanChunk - 1) & ~(kScanChunk - 1);
     L.max_ext = max_extensions(hp);
     Align a;
     L.scene = a.take(0);
@@ -651,12 +713,15 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
 void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
                        int grid, int lds_scene_f4, int max_ext, hipStream_t s) {
     const int D = robot_dof(robot);
-    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + (size_t)(max_ext + 6) * D * sizeof(float);
+    const int waves = plan_task_waves(robot);
+    // scene, chain rows, 4 uniform vectors, cross-wave buffers, leader decision
+    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) +
+                       ((size_t)(max_ext + 5) * D + 4 * kMaxTaskWaves + 4) * sizeof(float);
     switch (robot) {
 #define PRRTC_SLICE_CASE(ID, R)                                                                              \
     case ID:                                                                                                 \
-        hipLaunchKernelGGL(k_plan<R>, dim3(grid), dim3(kWave), lds, s, d_desc, d_state, d_ctrl, d_paths,     \
-                           lds_scene_f4, max_ext);                                                           \
+        hipLaunchKernelGGL(k_plan<R>, dim3(grid), dim3(waves * kWave), lds, s, d_desc, d_state, d_ctrl,      \
+                           d_paths, lds_scene_f4, max_ext);                                                  \
         break;
         PRRTC_SLICE_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
         PRRTC_SLICE_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
@@ -667,30 +732,47 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
     PRRTC_HIP(hipGetLastError());
 }
 
-// Workgroups per CU of each robot's persistent grid (prrtc_set_tuning).
+// Launch shape of each robot's persistent grid (prrtc_set_tuning; 0 = default).
 static std::atomic<int> g_wg_per_cu[4] = {{0}, {0}, {0}, {0}};
+static std::atomic<int> g_task_waves[4] = {{0}, {0}, {0}, {0}};
 
-static int default_wg_per_cu() {
-    static const int v = [] {
-        const char *e = std::getenv("PRRTC_WG_PER_CU");
-        const int n = e ? std::atoi(e) : 0;
-        return n >= 1 && n <= kMaxWgPerCu ? n : 8;
-    }();
-    return v;
+static int env_int(const char *name, int lo, int hi) {
+    const char *e = std::getenv(name);
+    const int n = e ? std::atoi(e) : 0;
+    return n >= lo && n <= hi ? n : 0;
+}
+
+int plan_task_waves(int robot) {
+    static const int env = env_int("PRRTC_TASK_WAVES", 1, kMaxTaskWaves);
+    const int v = robot >= 0 && robot < 4 ? g_task_waves[robot].load(std::memory_order_relaxed) : 0;
+    return v > 0 ? v : (env > 0 ? env : kDefaultTaskWaves);
 }
 
 int plan_wg_per_cu(int robot) {
+    static const int env = env_int("PRRTC_WG_PER_CU", 1, kMaxWgPerCu);
     const int v = robot >= 0 && robot < 4 ? g_wg_per_cu[robot].load(std::memory_order_relaxed) : 0;
-    return v > 0 ? v : default_wg_per_cu();
+    // default: 8 waves per CU
+    return v > 0 ? v : (env > 0 ? env : std::max(1, 8 / plan_task_waves(robot)));
 }
 
 // Solve problems that share one robot in one persistent launch.
-static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
-                        int path_stride, int64_t *kernel_ns, int *rounds) {
+// A robot group in flight: its problems and where its state lives.
+struct GroupRun {
+    std::vector<HostProblem> probs;
+    Workspace *ws = nullptr;
+    char *dev = nullptr, *host = nullptr;
+    size_t o_state = 0, o_ctrl = 0, o_paths = 0, h_read = 0;
+};
+
+// Upload a group, launch its persistent grid and queue the state readback
+// on the group's own stream (groups of different robots overlap).
+static void start_group(GroupRun &gr, int path_cap) {
+    const std::vector<HostProblem> &probs = gr.probs;
     const int P = (int)probs.size();
     const int robot = probs[0].robot;
     const int D = robot_dof(robot);
-    Workspace &ws = workspace();
+    Workspace &ws = workspace(robot);
+    gr.ws = &ws;
     // --- layout
     Align dl;
     const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
@@ -736,7 +818,7 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     // --- upload (descs + control in one copy; scenes and init rows per problem)
-    hipStream_t s = 0;
+    const hipStream_t s = ws.stream;
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
     for (int i = 0; i < P; i++)
         PRRTC_HIP(hipMemcpyAsync(dev + o_block[i], host + h_block[i], lay[i].upload_bytes, hipMemcpyHostToDevice, s));
@@ -745,32 +827,47 @@ static void solve_group(const std::vector<HostProblem> &probs, prrtc_result *res
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
     const int per_cu = plan_wg_per_cu(robot);
     const int grid = std::max(1, std::min(total_slots, num_cus() * per_cu));
-    hipEvent_t e0, e1;
-    PRRTC_HIP(hipEventCreate(&e0));
-    PRRTC_HIP(hipEventCreate(&e1));
-    PRRTC_HIP(hipEventRecord(e0, s));
+    PRRTC_HIP(hipEventRecord(ws.e0, s));
     const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
     PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
     float *d_paths = reinterpret_cast<float *>(dev + o_paths);
     launch_plan_init(robot, d_desc, d_state, nullptr, P, s);
     launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid, lds_f4, ext_all, s);
-    PRRTC_HIP(hipEventRecord(e1, s));
-    // --- read back states + control, then the used path rows
-    PlanState *hs = reinterpret_cast<PlanState *>(host + h_read);
-    PRRTC_HIP(hipMemcpyAsync(hs, dev + o_state, sizeof(PlanState) * P + 0, hipMemcpyDeviceToHost, s));
-    PlanControl *hctl = reinterpret_cast<PlanControl *>(host + h_read + sizeof(PlanState) * P);
-    PRRTC_HIP(hipMemcpyAsync(hctl, dev + o_ctrl, sizeof(PlanControl), hipMemcpyDeviceToHost, s));
+    PRRTC_HIP(hipEventRecord(ws.e1, s));
+    // --- queue the readback of states + control
+    PRRTC_HIP(hipMemcpyAsync(host + h_read, dev + o_state, sizeof(PlanState) * P, hipMemcpyDeviceToHost, s));
+    PRRTC_HIP(hipMemcpyAsync(host + h_read + sizeof(PlanState) * P, dev + o_ctrl, sizeof(PlanControl),
+                             hipMemcpyDeviceToHost, s));
+    gr.dev = dev;
+    gr.host = host;
+    gr.o_state = o_state;
+    gr.o_ctrl = o_ctrl;
+    gr.o_paths = o_paths;
+    gr.h_read = h_read;
+}
+
+// Wait for a started group, read the used path rows, write its results.
+static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int path_cap, int path_stride,
+                         int64_t *kernel_ns, int *rounds) {
+    const std::vector<HostProblem> &probs = gr.probs;
+    const int P = (int)probs.size();
+    const int D = robot_dof(probs[0].robot);
+    Workspace &ws = *gr.ws;
+    const hipStream_t s = ws.stream;
     PRRTC_HIP(hipStreamSynchronize(s));
     float ms = 0.0f;
-    PRRTC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *kernel_ns += (int64_t)(ms * 1e6);
+    PRRTC_HIP(hipEventElapsedTime(&ms, ws.e0, ws.e1));
+    *kernel_ns = std::max(*kernel_ns, (int64_t)(ms * 1e6));   // groups overlap: the longest one
+    const PlanState *hs = reinterpret_cast<const PlanState *>(gr.host + gr.h_read);
+    const PlanControl *hctl = reinterpret_cast<const PlanControl *>(gr.host + gr.h_read + sizeof(PlanState) * P);
     const int used = std::min(hctl->path_rows, hctl->path_cap);
     std::vector<float> rows((size_t)used * D);
-    if (used > 0)
-        PRRTC_HIP(hipMemcpy(rows.data(), d_paths, sizeof(float) * rows.size(), hipMemcpyDeviceToHost));
+    if (used > 0) {
+        PRRTC_HIP(hipMemcpyAsync(rows.data(), gr.dev + gr.o_paths, sizeof(float) * rows.size(),
+                                 hipMemcpyDeviceToHost, s));
+        PRRTC_HIP(hipStreamSynchronize(s));
+    }
     // --- results, path goal -> start (pRRTC.cu:715-746)
     for (int i = 0; i < P; i++) {
         const HostProblem &hp = probs[i];
@@ -835,7 +932,15 @@ static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, fl
     for (auto &hp : probs) groups[hp.robot].push_back(hp);
     int64_t kernel_ns = 0;
     int rounds = 0;
-    for (auto &kv : groups) solve_group(kv.second, results, paths, path_cap, path_stride, &kernel_ns, &rounds);
+    // every robot's group runs its persistent grid on its own stream; the
+    // groups overlap on the device
+    std::vector<GroupRun> runs(groups.size());
+    size_t k = 0;
+    for (auto &kv : groups) {
+        runs[k].probs = std::move(kv.second);
+        start_group(runs[k++], path_cap);
+    }
+    for (GroupRun &gr : runs) finish_group(gr, results, paths, path_cap, path_stride, &kernel_ns, &rounds);
     const int64_t wall = now_ns() - t0;
     int solved = 0;
     int64_t edges = 0;
@@ -873,12 +978,16 @@ int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, 
 }
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
-    if (!t || robot < -1 || robot > 3 || t->wg_per_cu < 0 || t->wg_per_cu > kMaxWgPerCu) {
+    if (!t || robot < -1 || robot > 3 || t->wg_per_cu < 0 || t->wg_per_cu > kMaxWgPerCu || t->waves_per_task < 0 ||
+        t->waves_per_task > kMaxTaskWaves) {
         set_error("prrtc_set_tuning: invalid argument");
         return PRRTC_EINVAL;
     }
-    for (int r = 0; r < 4; r++)
-        if (robot < 0 || r == robot) g_wg_per_cu[r].store(t->wg_per_cu, std::memory_order_relaxed);
+    for (int r = 0; r < 4; r++) {
+        if (robot >= 0 && r != robot) continue;
+        g_wg_per_cu[r].store(t->wg_per_cu, std::memory_order_relaxed);
+        g_task_waves[r].store(t->waves_per_task, std::memory_order_relaxed);
+    }
     return PRRTC_OK;
 }
 
@@ -888,6 +997,7 @@ int prrtc_get_tuning(int robot, prrtc_tuning *t) {
         return PRRTC_EINVAL;
     }
     t->wg_per_cu = plan_wg_per_cu(robot);
+    t->waves_per_task = plan_task_waves(robot);
     return PRRTC_OK;
 }
 

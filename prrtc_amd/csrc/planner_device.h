@@ -83,7 +83,14 @@ struct PlanControl {
 // slot outcome flags in meta[1]
 constexpr int kSkip = 1, kExtOk = 2, kConnected = 4;
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+// A planner task runs on one workgroup of 1..kMaxTaskWaves waves.
+constexpr int kMaxTaskWaves = 4;
+// Nearest-node scans read trees in chunks of this many nodes; tree storage
+// is a whole number of chunks.
+constexpr int kScanChunk = 1024;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
 
 __device__ __forceinline__ int atomic_add_agent(int *p, int v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -201,20 +208,26 @@ __device__ __forceinline__ int next_running(const unsigned long long *mask, int 
 }
 
 // Nearest node of tree t to q among [0, n): squared L2 as an fmaf chain in
-// dimension order (utils.cuh:100-109), ties to the smaller index.  The wave
-// strides the SoA coordinates (coalesced) and reduces by lane shuffles.
-// `tree` and `cap` keep every column 16-byte aligned (cap % 4 == 0).
+// dimension order (utils.cuh:100-109), ties to the smaller index.  The
+// workgroup's waves stride the SoA coordinates (coalesced) and reduce by lane
+// shuffles, then across waves through `red` (LDS, 2 * kMaxTaskWaves words).
+// `tree` and `cap` keep every column 16-byte aligned (cap % 4 == 0).  Every
+// thread of the workgroup calls it with the same arguments.
 template <int D>
-__device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq) {
+__device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq,
+                                       float *red) {
 #pragma clang fp contract(off)
     float best = __builtin_huge_valf();
     int idx = 0x7fffffff;
     const int lane = lane_id();
-    // 1024-node chunks: lane l reads nodes c + 256u + 4l .. +3 (u = 0..3) of
-    // every column as float4, so 4*D 16-byte loads are in flight per lane;
-    // each lane still visits its nodes in increasing index order
-    int c = 0;
-    for (; c + 16 * kWave <= n; c += 16 * kWave) {
+    const int W = blockDim.x / kWave, wv = wave_id();
+    // kScanChunk-node chunks, chunk k to wave k % W: lane l reads nodes
+    // c + 256u + 4l .. +3 (u = 0..3) of every column as float4, so 4*D 16-byte
+    // loads are in flight per lane.  The last chunk is read whole too (the
+    // tree's capacity is a multiple of kScanChunk) and nodes >= n are masked.
+    // Each lane visits its nodes in increasing index order.
+    static_assert(kScanChunk == 16 * kWave, "scan chunk = 16 nodes per lane");
+    for (int c = wv * kScanChunk; c < n; c += W * kScanChunk) {
         float acc[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) acc[e] = 0.0f;
@@ -237,23 +250,31 @@ __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const 
             }
         }
 #pragma unroll
-        for (int e = 0; e < 16; e++)
-            if (acc[e] < best) { best = acc[e]; idx = c + (e >> 2) * 4 * kWave + 4 * lane + (e & 3); }
-    }
-    for (int i = c + lane; i < n; i += kWave) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            const float diff = tree[(size_t)d * cap + i] - q[d];
-            acc = fmaf(diff, diff, acc);
+        for (int e = 0; e < 16; e++) {
+            const int i = c + (e >> 2) * 4 * kWave + 4 * lane + (e & 3);
+            if (i < n && acc[e] < best) { best = acc[e]; idx = i; }
         }
-        if (acc < best) { best = acc; idx = i; }
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
         const float ob = __shfl_xor(best, off);
         const int oi = __shfl_xor(idx, off);
         if (ob < best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+    }
+    if (W > 1) {
+        if (lane == 0) {
+            red[2 * wv] = best;
+            red[2 * wv + 1] = __int_as_float(idx);
+        }
+        __syncthreads();
+        best = red[0];
+        idx = __float_as_int(red[1]);
+        for (int w = 1; w < W; w++) {
+            const float ob = red[2 * w];
+            const int oi = __float_as_int(red[2 * w + 1]);
+            if (ob < best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+        }
+        __syncthreads();   // `red` may be written again right after
     }
     best_sq = best;
     return idx;

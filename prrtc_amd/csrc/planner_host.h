@@ -40,10 +40,14 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
 // Fills a PlanDesc whose buffers live in the device block at `dev`.
 void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev);
 
-// Single-wave workgroups per CU of a robot's persistent planner grid: the
-// value set by prrtc_set_tuning, else the environment variable
-// PRRTC_WG_PER_CU (1..32), else 8.
+// Launch shape of a robot's persistent planner grid: the value set by
+// prrtc_set_tuning, else an environment variable, else the default.
+//   plan_task_waves  waves per task workgroup, 1..4 (PRRTC_TASK_WAVES)
+//   plan_wg_per_cu   workgroups per CU, 1..32 (PRRTC_WG_PER_CU; default
+//                    8 / waves)
 constexpr int kMaxWgPerCu = 32;
+constexpr int kDefaultTaskWaves = 2;
+int plan_task_waves(int robot);
 int plan_wg_per_cu(int robot);
 
 // Tree roots and Halton states of the problems listed in d_list (table

@@ -67,7 +67,19 @@ __global__ __launch_bounds__(kWave) void k_validate_edges(const float *__restric
     const int lane = threadIdx.x;
     const int passes = (g + kWave - 1) / kWave;
     const float gf = (float)g;
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    // Workgroups reach the XCDs round-robin (workgroup b runs on XCD b % 8),
+    // and each XCD has its own L2.  Giving the workgroups of one XCD one
+    // contiguous range of edges keeps every cache line of the edge list in
+    // one L2 (edges are 4*2*D bytes and straddle lines).
+    int lo = 0, hi = n, first = blockIdx.x, stride = gridDim.x;
+    if ((int)gridDim.x >= kXcds) {
+        const int x = blockIdx.x % kXcds;
+        lo = (int)((long long)n * x / kXcds);
+        hi = (int)((long long)n * (x + 1) / kXcds);
+        first = blockIdx.x / kXcds;
+        stride = ((int)gridDim.x - x + kXcds - 1) / kXcds;   // workgroups on this XCD
+    }
+    for (int e = lo + first; e < hi; e += stride) {
         const float *a = edges + (size_t)e * 2 * D;
         const float *b = a + D;
         float av[D], dv[D];

@@ -66,7 +66,7 @@ class CBatchStats(C.Structure):
 
 
 class CTuning(C.Structure):
-    _fields_ = [("wg_per_cu", C.c_int32)]
+    _fields_ = [("wg_per_cu", C.c_int32), ("waves_per_task", C.c_int32)]
 
 
 class CServerConfig(C.Structure):
@@ -184,17 +184,17 @@ def settings_default() -> dict:
     return {name: getattr(st, name) for name, _ in CSettings._fields_}
 
 
-def set_tuning(robot, wg_per_cu: int) -> None:
-    """Workgroups per CU of the persistent planner grid for `robot` (or every
-    robot when robot is None); 0 restores the default."""
-    t = CTuning(int(wg_per_cu))
+def set_tuning(robot, wg_per_cu: int = 0, waves_per_task: int = 0) -> None:
+    """Launch shape of the persistent planner for `robot` (every robot when
+    robot is None): task workgroups per CU and waves per task; 0 = default."""
+    t = CTuning(int(wg_per_cu), int(waves_per_task))
     _check(load().prrtc_set_tuning(-1 if robot is None else robot_id(robot), C.byref(t)), "prrtc_set_tuning")
 
 
 def get_tuning(robot) -> dict:
     t = CTuning()
     _check(load().prrtc_get_tuning(robot_id(robot), C.byref(t)), "prrtc_get_tuning")
-    return {"wg_per_cu": t.wg_per_cu}
+    return {"wg_per_cu": t.wg_per_cu, "waves_per_task": t.waves_per_task}
 
 
 def pack_scene(scene: Scene) -> np.ndarray:
