@@ -111,7 +111,7 @@ def main():
     g = args.granularity
     tuned = load_tuned()
     for robot, t in tuned.get("launch", {}).items():
-        P.set_tuning(robot, t.get("wg_per_cu", 0), t.get("waves_per_task", 0))
+        P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0))
 
     # ---------------------------------------------------------------- inputs
     log("building the scene and the edges")
@@ -204,6 +204,36 @@ def main():
                      "flops_per_pose": per_pose, "poses_per_launch": n_roof * g},
     }
     paths_to_check = []   # (robot, scene, path, g)
+
+    # ------------------------------------------------- nearest-node search
+    # 256 Fetch-sized trees of 100k nodes (SoA, 822 MB: beyond L2 and MALL),
+    # 64 queries per tree (one planner round of nnc 64); bytes per query =
+    # 4 * dof * n (every coordinate read once)
+    n_trees, n_nodes, qpt, ndof = 256, 100_000, 64, 8
+    ncap = ((n_nodes + 1023) // 1024) * 1024
+    gen = torch.Generator(device=dev).manual_seed(3)
+    d_trees = torch.rand((n_trees, ndof, ncap), generator=gen, device=dev)
+    d_q = torch.rand((n_trees, qpt, ndof), generator=gen, device=dev)
+    d_idx = torch.zeros(n_trees * qpt, dtype=torch.int32, device=dev)
+    d_d2 = torch.zeros(n_trees * qpt, dtype=torch.float32, device=dev)
+    nn_waves = tuned.get("nearest_waves", 4)
+
+    def nn_step():
+        P.nearest_async(ndof, d_trees.data_ptr(), ncap, n_nodes, n_trees, d_q.data_ptr(), qpt, nn_waves,
+                        d_idx.data_ptr(), d_d2.data_ptr(), stream)
+
+    nn_step()
+    nn_ms = time_launches(nn_step, args.steps, torch)
+    nn_bytes = 4.0 * ndof * n_nodes * n_trees * qpt
+    out["nearest"] = {"trees": n_trees, "nodes": n_nodes, "dof": ndof, "queries_per_tree": qpt,
+                      "waves_per_query": nn_waves, "kernel_ms": nn_ms,
+                      "algorithmic_GBps": nn_bytes / (nn_ms / 1e3) / 1e9,
+                      "frac_of_hbm_peak": nn_bytes / (nn_ms / 1e3) / HBM_PEAK,
+                      "unique_tree_bytes": 4.0 * ndof * ncap * n_trees,
+                      "note": "queries of one tree share its reads through L2; profiles/ holds the HBM counters"}
+    del d_trees
+    log(f"nearest: {nn_ms:.3f} ms, {out['nearest']['algorithmic_GBps']:.0f} GB/s algorithmic "
+        f"({out['nearest']['frac_of_hbm_peak'] * 100:.1f}% of 8 TB/s)")
 
     if not args.skip_plans:
         # ------------------------------------------------ plan (same scene)

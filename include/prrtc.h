@@ -113,6 +113,17 @@ int prrtc_validate_edges(int robot, const prrtc_scene *scene, const float *edges
 int prrtc_validate_edges_async(int robot, const float *d_scene, size_t scene_floats, const float *d_edges, int n,
                                int granularity, uint8_t *d_out_free, void *stream);
 
+/* ---- nearest-node search (replaces the tree scans of pRRTC.cu:384-423 and
+ *      478-514, distance utils.cuh:89-109), device buffers, asynchronous ----
+ * n_trees SoA trees, tree t at d_trees + t * dof * cap: coordinate k of node
+ * i at [k * cap + i], nodes [0, n_nodes) valid, cap a multiple of 1024.
+ * Query j of tree t is d_queries[(t * queries_per_tree + j) * dof ..]; its
+ * answer (smallest squared L2 distance, ties to the smaller index) goes to
+ * d_idx / d_dist2 at t * queries_per_tree + j.  waves (1..4) wavefronts
+ * share one query.  dof is 3, 7, 8 or 14. */
+int prrtc_nearest_async(int dof, const float *d_trees, int cap, int n_nodes, int n_trees, const float *d_queries,
+                        int queries_per_tree, int waves, int *d_idx, float *d_dist2, void *stream);
+
 /* ---- planner (replaces pRRTC::solve<Robot>, src/planning/Planners.hh:75-78,
  *      defined at src/planning/pRRTC.cu:607-766) ----
  * path: path_cap rows of dof floats, written goal -> start. */
@@ -148,16 +159,22 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
 /* Launch shape of the persistent planner per robot (no reference
  * counterpart: the reference launches num_new_configs blocks of 128 threads
  * per problem, src/planning/pRRTC.cu:700-702).  0 = default for each field.
- *   waves_per_task  waves (64 lanes each) that share one planner task: its
- *                   nearest-node scans and its edge / connect-chain poses,
- *                   1..4 (environment PRRTC_TASK_WAVES, else 2)
- *   wg_per_cu       task workgroups per CU of the persistent grid, 1..32
- *                   (environment PRRTC_WG_PER_CU, else 8 / waves_per_task)
+ *   waves_per_cu          waves per CU of a persistent grid, 1..32
+ *                         (environment PRRTC_WAVES_PER_CU, else 8)
+ *   waves_per_task        waves (64 lanes each) sharing one planner task in
+ *                         the latency shape -- single solves, a server with
+ *                         few resident problems, the last problems of a
+ *                         batch: they share the nearest-node scans and split
+ *                         the obstacle tables of every validation pass
+ *                         (1..4; environment PRRTC_TASK_WAVES, else 4)
+ *   batch_waves_per_task  the same for the throughput shape, while many
+ *                         problems run (1..4; PRRTC_BATCH_WAVES, else 4)
  * robot = -1 sets every robot.  Applies to later prrtc_solve,
  * prrtc_solve_batch and server slices; results do not depend on it. */
 typedef struct prrtc_tuning {
-    int32_t wg_per_cu;
+    int32_t waves_per_cu;
     int32_t waves_per_task;
+    int32_t batch_waves_per_task;
 } prrtc_tuning;
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t);

@@ -1,0 +1,133 @@
+"""Autotuner of the planner's launch shapes and num_new_configs per robot class.
+
+    python -m prrtc_amd.autotune [--robots fetch,panda,baxter] [--problems 96] [--out PATH]
+
+For every robot class it builds a seeded problem set of the benchmark's kind
+(Fetch: 8-64 mixed obstacles, g 64; Panda and Baxter: shelves with 4-128
+obstacles, g 128, dynamic domain), keeps the problems a pilot run connects,
+and measures on the GPU:
+
+* batched form (one persistent launch): plans/s over num_new_configs x waves
+  per task x waves per CU; the best sets ``batch_nnc`` and the launch shape;
+* plain form (one problem per launch): plans/s over num_new_configs x waves
+  per task on a subset; the best sets ``plain_nnc`` and the latency shape.
+
+A lane always carries one pose (64 poses per wavefront); the tuned shape is
+how many wavefronts share one task (they split the obstacle tables of each
+validation pass) and how many wavefronts a CU holds.  Launch shapes never
+change results; num_new_configs does (it is a planner setting), so every
+candidate is timed on the same problems and the choice is by solved plans
+per second.  The result is JSON, read by bench.py:
+
+    {"launch": {robot: {"waves_per_cu", "waves_per_task", "batch_waves_per_task"}},
+     "batch_nnc": {robot: n}, "plain_nnc": {robot: n}, "serve_nnc": {robot: n},
+     "measured": {...every candidate's plans/s...}}
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "tuned.json")
+
+SETTINGS = {
+    "fetch": dict(granularity=64, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0, max_samples=100000,
+                  seed=1),
+    "panda": dict(granularity=128, range=0.5, balance=1, tree_ratio=0.5, dynamic_domain=1, dd_radius=4.0,
+                  dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, seed=1),
+    "baxter": dict(granularity=128, range=0.5, balance=1, tree_ratio=0.5, dynamic_domain=1, dd_radius=4.0,
+                   dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, seed=1),
+}
+
+
+def log(msg):
+    print(f"[autotune] {msg}", file=sys.stderr, flush=True)
+
+
+def problems(robot, n):
+    from . import workloads
+    if robot == "fetch":
+        return workloads.c2_problems(n, seed=0)
+    counts = {"panda": (0, n, 0), "baxter": (n, 0, 0)}[robot]
+    return workloads.c3_problems(*counts, seed=0)
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return time.perf_counter() - t0, out
+
+
+def tune_robot(P, robot, n, pilot_iters, max_iters, plain_n):
+    base = SETTINGS[robot]
+    probs = problems(robot, n)
+    _, (pres, _, _) = timed(lambda: P.solve_batch(probs, dict(base, num_new_configs=64, max_iters=pilot_iters)))
+    feas = [p for p, r in zip(probs, pres) if r["solved"]]
+    log(f"{robot}: {len(feas)}/{len(probs)} problems connect within {pilot_iters} rounds")
+    measured = {"batch": [], "plain": []}
+    best_b = None
+    for waves in (1, 2, 4):
+        for wpc in (4, 8):
+            if wpc < waves:
+                continue
+            P.set_tuning(robot, waves_per_cu=wpc, waves_per_task=waves, batch_waves_per_task=waves)
+            for nnc in (32, 64, 128, 256):
+                st = dict(base, num_new_configs=nnc, max_iters=max_iters)
+                dt, (res, _, _) = timed(lambda: P.solve_batch(feas, st))
+                rate = sum(r["solved"] for r in res) / dt
+                measured["batch"].append({"waves_per_task": waves, "waves_per_cu": wpc, "nnc": nnc,
+                                          "plans_per_s": rate, "wall_s": dt})
+                log(f"{robot} batch waves {waves} waves/CU {wpc} nnc {nnc}: {rate:.1f} plans/s")
+                if best_b is None or rate > best_b[0]:
+                    best_b = (rate, waves, wpc, nnc)
+    sub = feas[:plain_n]
+    best_p = None
+    for waves in (2, 4):
+        P.set_tuning(robot, waves_per_cu=best_b[2], waves_per_task=waves, batch_waves_per_task=best_b[1])
+        for nnc in (64, 128, 256, 512):
+            st = dict(base, num_new_configs=nnc, max_iters=max_iters)
+            dt, res = timed(lambda: [P.solve(p["robot"], p["start"], p["goals"], p["scene"], st) for p in sub])
+            rate = sum(r[0]["solved"] for r in res) / dt
+            measured["plain"].append({"waves_per_task": waves, "nnc": nnc, "plans_per_s": rate, "wall_s": dt})
+            log(f"{robot} plain waves {waves} nnc {nnc}: {rate:.1f} plans/s")
+            if best_p is None or rate > best_p[0]:
+                best_p = (rate, waves, nnc)
+    P.set_tuning(robot, 0, 0, 0)
+    launch = {"waves_per_cu": best_b[2], "waves_per_task": best_p[1], "batch_waves_per_task": best_b[1]}
+    return launch, best_b[3], best_p[2], measured
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--robots", default="fetch,panda,baxter")
+    ap.add_argument("--problems", type=int, default=96)
+    ap.add_argument("--plain-problems", type=int, default=24)
+    ap.add_argument("--pilot-iters", type=int, default=2000)
+    ap.add_argument("--max-iters", type=int, default=20000)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    args = ap.parse_args(argv)
+    import prrtc_amd as P
+    P.load()
+    out = {"launch": {}, "batch_nnc": {}, "plain_nnc": {}, "serve_nnc": {}, "measured": {},
+           "problems": args.problems}
+    for robot in args.robots.split(","):
+        n = args.problems if robot != "baxter" else max(8, args.problems // 3)
+        launch, nb, npl, measured = tune_robot(P, robot, n, args.pilot_iters, args.max_iters, args.plain_problems)
+        out["launch"][robot] = launch
+        out["batch_nnc"][robot] = nb
+        out["plain_nnc"][robot] = npl
+        out["serve_nnc"][robot] = min(nb, npl)
+        out["measured"][robot] = measured
+        log(f"{robot}: launch {launch}, batch nnc {nb}, plain nnc {npl}")
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"wrote {args.out}")
+
+
+if __name__ == "__main__":
+    main()

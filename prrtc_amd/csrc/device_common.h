@@ -53,6 +53,25 @@ struct Obstacles {
         cub = zcap + 2 * n_zcap;
         zcub = cub + 4 * n_cub;
     }
+
+    // Slice `part` of `parts`: a contiguous share of every table (the slices
+    // of one table partition it).
+    __device__ Obstacles slice(int part, int parts) const {
+        Obstacles o = *this;
+        cut(o.sph, o.n_sph, 1, part, parts);
+        cut(o.cap, o.n_cap, 2, part, parts);
+        cut(o.zcap, o.n_zcap, 2, part, parts);
+        cut(o.cub, o.n_cub, 4, part, parts);
+        cut(o.zcub, o.n_zcub, 4, part, parts);
+        return o;
+    }
+
+private:
+    __device__ static void cut(const float4 *&p, int &n, int stride, int part, int parts) {
+        const int lo = n * part / parts, hi = n * (part + 1) / parts;
+        p += (size_t)lo * stride;
+        n = hi - lo;
+    }
 };
 
 // Read a packed scene's header.  `cap_f4` is the payload size the host
@@ -214,6 +233,10 @@ struct CheckCtx {
         return self_hit(ax, ay, az, bx, by, bz, rs);
     }
     __device__ __forceinline__ void hit() { coll = true; }
+    // Gate bits of this lane's pose after the gate pass.  A context whose
+    // waves split the obstacle tables ORs them across its waves; here every
+    // wave sees every obstacle, so there is nothing to share.
+    __device__ __forceinline__ void share_gates(uint64_t &) const {}
     __device__ __forceinline__ bool stop() {
         if (!independent) {
             const uint64_t cm = ballot(coll);

@@ -37,22 +37,45 @@ __device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int 
     return r;
 }
 
+// Check context of a planner task.  The task's waves validate the same 64
+// poses per pass and split the obstacle tables between them (each wave's
+// `obs` is one slice), so a pass costs a fraction of one wave's work; a
+// pose's gate bits are ORed across the waves before the leaf checks, and
+// the waves' collision flags are combined after the pass.  A gate hits iff
+// some obstacle hits it, whichever wave tests that obstacle, so the verdicts
+// are those of a single wave that sees every obstacle.
+struct TaskCtx : CheckCtx {
+    uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
+    __device__ __forceinline__ void share_gates(uint64_t &gate) {
+        const int W = blockDim.x / kWave;
+        if (W == 1) return;
+        gbuf[threadIdx.x] = gate;
+        __syncthreads();
+        uint64_t g = 0;
+        const int lane = lane_id();
+        for (int w = 0; w < W; w++) g |= gbuf[w * kWave + lane];
+        __syncthreads();   // gbuf is written again by the next pass
+        gate = g;
+    }
+};
+
 // One task: slot s of problem p in the current round (pRRTC.cu:342-551),
 // run by every thread of the workgroup (1..kMaxTaskWaves waves).
 //
 // The steered edge and the connect chain are validated by one loop with a
 // single call site of the robot's check sequence (the sequence is large; one
-// inlined copy keeps register use down); its poses are spread over all
-// threads.  Uniform vectors live in LDS: chain[i] holds the chain
-// configurations c_0 = new node, c_{i+1} = c_i + vec; uv holds the nearest
-// node, the steered step and the connect step; red and wbuf are the
-// cross-wave reduction buffers.
+// inlined copy keeps register use down); a pass puts 64 poses on the lanes
+// of every wave, the waves splitting the obstacles (TaskCtx).  Uniform
+// vectors live in LDS: chain[i] holds the chain configurations c_0 = new
+// node, c_{i+1} = c_i + vec; uv holds the nearest node, the steered step and
+// the connect step; red and wbuf are the cross-wave reduction buffers.
 template <class Robot>
-__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx, float *chain, float *uv,
+__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx, float *chain, float *uv,
                           float *red, int *wbuf) {
 #pragma clang fp contract(off)
     constexpr int D = Robot::DOF;
     const int tid = threadIdx.x, NT = blockDim.x;
+    const int lane = lane_id();
     const int iter = st->round;
     const int n0 = st->size[0], n1 = st->size[1];
     const int g = d.g;
@@ -107,7 +130,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
         __syncthreads();
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
-        const int passes = (g + NT - 1) / NT;
+        const int passes = (g + kWave - 1) / kWave;
         int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0;
         unsigned n_ext = 0;
         edges = 1;
@@ -116,13 +139,13 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
             int key = 0;
             bool dead;
             if (phase == 0) {
-                const int k = tid * passes + p;
+                const int k = lane * passes + p;
                 const float kk = (float)((k < g ? k : g - 1) + 1);
 #pragma unroll
                 for (int c = 0; c < D; c++) q[c] = fmaf(kk, delta[c], nnv[c]);
                 dead = k >= g;
             } else {
-                const int j = base + tid;
+                const int j = base + lane;
                 const int jj = j < total ? j : total - 1;
                 const int i = jj / g, k = jj % g;
 #pragma unroll
@@ -136,14 +159,14 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, CheckCtx &ctx
             Robot::check(q, ctx);
             const uint64_t cm = ballot(ctx.coll);
             if (phase == 1) {
-                // first colliding sub-edge: waves hold increasing pose ranges
+                // first colliding sub-edge: the smallest any wave found
                 const int wfirst = cm ? __builtin_amdgcn_readlane(key, __builtin_ctzll(cm)) : 0x7fffffff;
                 const int first = across_waves(wfirst, true, wbuf, npass++ & 1);
                 if (first != 0x7fffffff) {
                     n_ok = first;
                     break;
                 }
-                base += NT;
+                base += kWave;
                 if (base >= total) { n_ok = n_chain; break; }
                 continue;
             }
@@ -363,15 +386,20 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
     status = __builtin_amdgcn_readfirstlane(status);
     __threadfence();
     if (lane == 0) {
+        const unsigned long long key = ((unsigned long long)st->round << 32) | (unsigned)p;
         if (status == kStatusRunning) {
             st->round = st->round + 1;
             st->done_slots = 0;
             store_release(&st->next_slot, 0);
+            __hip_atomic_fetch_max(&ctrl->hot_key, key + (1ull << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             store_release(&st->next_slot, kClaimedOut);
             __hip_atomic_fetch_and(ctrl->run_mask + (p >> 6), ~(1ull << (p & 63)), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            atomic_add_agent(&ctrl->finished, 1);
+            // a finished problem hands the priority back to the next commit
+            unsigned long long expect = key;
+            __hip_atomic_compare_exchange_strong(&ctrl->hot_key, &expect, 0ull, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     return status;
@@ -382,18 +410,23 @@ __global__ __launch_bounds__(kMaxTaskWaves * kWave) __attribute__((amdgpu_waves_
 void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *ctrl, float *paths, int lds_scene_f4,
             int lds_ext) {
     extern __shared__ float4 lds[];
-    float *chain = reinterpret_cast<float *>(lds + lds_scene_f4);
+    // LDS: scene tables | gate exchange | chain rows | uniform vectors |
+    // cross-wave buffers | the leader's decision
+    uint64_t *gbuf = reinterpret_cast<uint64_t *>(lds + lds_scene_f4);       // kMaxTaskWaves * kWave
+    float *chain = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);
     float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
     float *red = uv + 4 * Robot::DOF;                                 // 2 * kMaxTaskWaves
     int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);     // 2 * kMaxTaskWaves
     int *ctl = wbuf + 2 * kMaxTaskWaves;                              // the leader's decision: p, s
-    CheckCtx ctx;
+    TaskCtx ctx;
     ctx.independent = false;
+    ctx.gbuf = gbuf;
     const bool leader = wave_id() == 0;   // wave 0 claims the tasks and commits the rounds
     int staged = -1;
     int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
-    const int R = ctrl->n_running;
+    const int min_running = ctrl->min_running;
+    const int prio_policy = ctrl->prio_policy;
     const int max_nnc = ctrl->max_nnc;
     const unsigned long long slice = ctrl->slice_ticks;
     unsigned long long t0 = 0;
@@ -413,35 +446,51 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
     int cursor = (int)(blockIdx.x % (unsigned)P);   // each workgroup starts at its own problem
     while (true) {
         if (leader) {
-            int p_out = -1, s_out = -1;   // p_out < 0: the workgroup leaves
+            int p_out = -1, s_out = -1, hot = 0;   // p_out < 0: the workgroup leaves
+            // Claim slot s of problem q if q runs and has unclaimed slots in
+            // this round.  Plain loads come first, so the counters' cache
+            // lines stay cheap to share; a claim counts only below nnc and
+            // finished problems are never incremented, so it cannot wrap.
+            auto try_claim = [&](int q) -> int {
+                PlanState *st = states + q;
+                const int nnc = descs[q].nnc;
+                const bool live = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
+                const bool open = live && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                              &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
+                const int s = open ? wave_atomic_add(&st->next_slot, 1) : nnc;
+                return s >= 0 && s < nnc ? s : -1;
+            };
             while (true) {
-                const int fin = load_acquire(&ctrl->finished);
-                // the running problems can use at most (R - fin) * nnc
+                // the next running problem from the cursor on (finished ones
+                // are skipped through the mask, not visited one by one)
+                int running = 0;
+                const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor, running));
+                // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
-                // share their counters with the whole grid
-                if (fin >= R || (long long)blockIdx.x >= (long long)(R - fin) * max_nnc) break;
+                // share their counters with the whole grid.  A launch with
+                // min_running > 0 ends once that few problems are left (the
+                // next launch, shaped for latency, finishes them).
+                if (running <= min_running || (long long)blockIdx.x >= (long long)running * max_nnc) break;
                 // a time-sliced launch ends between tasks; unfinished rounds
                 // resume in the next launch from the state kept in device memory
                 if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
-                // the next running problem from the cursor on (finished ones
-                // are skipped through the mask, not visited one by one)
-                const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor));
-                int s = -1, nnc = 0;
-                if (p >= 0) {
-                    PlanState *st = states + p;
-                    nnc = descs[p].nnc;
-                    // Only a problem that is running and still has unclaimed
-                    // slots in this round is touched with an atomic (plain
-                    // loads first keep the counters' cache lines cheap to
-                    // share).  A claim counts only below nnc, and finished
-                    // problems are never incremented, so the counter cannot wrap.
-                    const bool running =
-                        __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
-                    const bool open = running && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                                                     &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
-                    if (open) s = wave_atomic_add(&st->next_slot, 1);
+                // the problem furthest along first
+                const unsigned long long hk = __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+                const int ph = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
+                if (ph >= 0 && ph < P) {
+                    const int s = try_claim(ph);
+                    if (s >= 0) {
+                        p_out = ph;
+                        s_out = s;
+                        hot = 1;
+                        idle = 0;
+                        backoff = 1;
+                        break;
+                    }
                 }
-                if (s >= 0 && s < nnc) {
+                const int s = try_claim(p);
+                if (s >= 0) {
                     p_out = p;
                     s_out = s;
                     idle = 0;
@@ -450,25 +499,35 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 }
                 // move on; after visiting every running problem without work,
                 // sleep (exponential back-off, up to 4 x 64 x 64 cycles)
-                if (p >= 0) cursor = p + 1 < P ? p + 1 : 0;
-                if (p < 0 || ++idle >= R - fin) {
+                cursor = p + 1 < P ? p + 1 : 0;
+                if (++idle >= running) {
                     for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
                     backoff = backoff < 4 ? 2 * backoff : 4;
                     idle = 0;
                 }
             }
+            const int prio = p_out >= 0 ? task_priority(prio_policy, states[p_out].round, hot != 0) : 0;
             if (lane_id() == 0) {
                 ctl[0] = p_out;
                 ctl[1] = s_out;
+                ctl[2] = prio;
             }
         }
         __syncthreads();
         const int p = ctl[0], s = ctl[1];
         if (p < 0) break;
+        // issue priority of this task's waves on their SIMDs
+        switch (ctl[2]) {
+            case 3: __builtin_amdgcn_s_setprio(3); break;
+            case 2: __builtin_amdgcn_s_setprio(2); break;
+            case 1: __builtin_amdgcn_s_setprio(1); break;
+            default: __builtin_amdgcn_s_setprio(0); break;
+        }
         const PlanDesc &d = descs[p];
         PlanState *st = states + p;
         if (staged != p) {   // every wave is past its last use of the staged scene
             bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
+            ctx.obs = ctx.obs.slice(wave_id(), blockDim.x / kWave);   // this wave's share of the obstacles
             staged = p;
         }
         plan_task<Robot>(d, st, s, ctx, chain, uv, red, wbuf);
@@ -711,11 +770,11 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
 }
 
 void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
-                       int grid, int lds_scene_f4, int max_ext, hipStream_t s) {
+                       int grid, int waves, int lds_scene_f4, int max_ext, hipStream_t s) {
     const int D = robot_dof(robot);
-    const int waves = plan_task_waves(robot);
-    // scene, chain rows, 4 uniform vectors, cross-wave buffers, leader decision
-    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) +
+    // scene, gate exchange, chain rows, 4 uniform vectors, cross-wave
+    // buffers, leader decision
+    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + kMaxTaskWaves * kWave * sizeof(uint64_t) +
                        ((size_t)(max_ext + 5) * D + 4 * kMaxTaskWaves + 4) * sizeof(float);
     switch (robot) {
 #define PRRTC_SLICE_CASE(ID, R)                                                                              \
@@ -732,9 +791,10 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
     PRRTC_HIP(hipGetLastError());
 }
 
-// Launch shape of each robot's persistent grid (prrtc_set_tuning; 0 = default).
-static std::atomic<int> g_wg_per_cu[4] = {{0}, {0}, {0}, {0}};
+// Launch shape of each robot's persistent grids (prrtc_set_tuning; 0 = default).
+static std::atomic<int> g_waves_per_cu[4] = {{0}, {0}, {0}, {0}};
 static std::atomic<int> g_task_waves[4] = {{0}, {0}, {0}, {0}};
+static std::atomic<int> g_batch_waves[4] = {{0}, {0}, {0}, {0}};
 
 static int env_int(const char *name, int lo, int hi) {
     const char *e = std::getenv(name);
@@ -742,17 +802,29 @@ static int env_int(const char *name, int lo, int hi) {
     return n >= lo && n <= hi ? n : 0;
 }
 
-int plan_task_waves(int robot) {
-    static const int env = env_int("PRRTC_TASK_WAVES", 1, kMaxTaskWaves);
-    const int v = robot >= 0 && robot < 4 ? g_task_waves[robot].load(std::memory_order_relaxed) : 0;
-    return v > 0 ? v : (env > 0 ? env : kDefaultTaskWaves);
+static int tuned(const std::atomic<int> *table, int robot, int env, int dflt) {
+    const int v = robot >= 0 && robot < 4 ? table[robot].load(std::memory_order_relaxed) : 0;
+    return v > 0 ? v : (env > 0 ? env : dflt);
 }
 
-int plan_wg_per_cu(int robot) {
-    static const int env = env_int("PRRTC_WG_PER_CU", 1, kMaxWgPerCu);
-    const int v = robot >= 0 && robot < 4 ? g_wg_per_cu[robot].load(std::memory_order_relaxed) : 0;
-    // default: 8 waves per CU
-    return v > 0 ? v : (env > 0 ? env : std::max(1, 8 / plan_task_waves(robot)));
+int plan_task_waves(int robot) {
+    static const int env = env_int("PRRTC_TASK_WAVES", 1, kMaxTaskWaves);
+    return tuned(g_task_waves, robot, env, kDefaultTaskWaves);
+}
+
+int plan_batch_waves(int robot) {
+    static const int env = env_int("PRRTC_BATCH_WAVES", 1, kMaxTaskWaves);
+    return tuned(g_batch_waves, robot, env, kDefaultBatchWaves);
+}
+
+int plan_waves_per_cu(int robot) {
+    static const int env = env_int("PRRTC_WAVES_PER_CU", 1, kMaxWavesPerCu);
+    return tuned(g_waves_per_cu, robot, env, kDefaultWavesPerCu);
+}
+
+int plan_grid(int robot, int waves, long long slots) {
+    const long long per_cu = std::max(1, plan_waves_per_cu(robot) / waves);
+    return (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * per_cu));
 }
 
 // Solve problems that share one robot in one persistent launch.
@@ -810,7 +882,6 @@ static void start_group(GroupRun &gr, int path_cap) {
     PlanControl *hc = reinterpret_cast<PlanControl *>(host + h_hdr + o_ctrl);
     *hc = PlanControl{};
     hc->n_problems = P;
-    hc->n_running = P;
     hc->mask_words = words;
     hc->run_mask = reinterpret_cast<unsigned long long *>(dev + o_mask);
     unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + h_hdr + o_mask);
@@ -822,18 +893,30 @@ static void start_group(GroupRun &gr, int path_cap) {
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
     for (int i = 0; i < P; i++)
         PRRTC_HIP(hipMemcpyAsync(dev + o_block[i], host + h_block[i], lay[i].upload_bytes, hipMemcpyHostToDevice, s));
-    // --- launch
-    int total_slots = 0;
+    // --- launch shapes: a throughput grid (batch_waves per task) while many
+    // problems run, then a latency grid (task waves, obstacles split between
+    // them) for the last few; both launches are queued back to back and the
+    // first leaves once at most `tail` problems run
+    long long total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
-    const int per_cu = plan_wg_per_cu(robot);
-    const int grid = std::max(1, std::min(total_slots, num_cus() * per_cu));
-    PRRTC_HIP(hipEventRecord(ws.e0, s));
+    const int w_lat = plan_task_waves(robot), w_thr = plan_batch_waves(robot);
+    const int grid_lat = plan_grid(robot, w_lat, total_slots);
+    const int tail = std::max(1, grid_lat / std::max(1, hc->max_nnc));
+    const bool two_phase = w_thr != w_lat && P > tail;
+    hc->min_running = two_phase ? tail : 0;
+    hc->prio_policy = kPrioOldest;
     const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
     PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
     float *d_paths = reinterpret_cast<float *>(dev + o_paths);
+    PRRTC_HIP(hipEventRecord(ws.e0, s));
     launch_plan_init(robot, d_desc, d_state, nullptr, P, s);
-    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid, lds_f4, ext_all, s);
+    if (two_phase) {
+        launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, w_thr, total_slots), w_thr,
+                          lds_f4, ext_all, s);
+        PRRTC_HIP(hipMemsetAsync(&d_ctrl->min_running, 0, sizeof(int), s));
+    }
+    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid_lat, w_lat, lds_f4, ext_all, s);
     PRRTC_HIP(hipEventRecord(ws.e1, s));
     // --- queue the readback of states + control
     PRRTC_HIP(hipMemcpyAsync(host + h_read, dev + o_state, sizeof(PlanState) * P, hipMemcpyDeviceToHost, s));
@@ -978,15 +1061,17 @@ int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, 
 }
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
-    if (!t || robot < -1 || robot > 3 || t->wg_per_cu < 0 || t->wg_per_cu > kMaxWgPerCu || t->waves_per_task < 0 ||
-        t->waves_per_task > kMaxTaskWaves) {
+    auto bad = [](int v, int hi) { return v < 0 || v > hi; };
+    if (!t || robot < -1 || robot > 3 || bad(t->waves_per_cu, kMaxWavesPerCu) ||
+        bad(t->waves_per_task, kMaxTaskWaves) || bad(t->batch_waves_per_task, kMaxTaskWaves)) {
         set_error("prrtc_set_tuning: invalid argument");
         return PRRTC_EINVAL;
     }
     for (int r = 0; r < 4; r++) {
         if (robot >= 0 && r != robot) continue;
-        g_wg_per_cu[r].store(t->wg_per_cu, std::memory_order_relaxed);
+        g_waves_per_cu[r].store(t->waves_per_cu, std::memory_order_relaxed);
         g_task_waves[r].store(t->waves_per_task, std::memory_order_relaxed);
+        g_batch_waves[r].store(t->batch_waves_per_task, std::memory_order_relaxed);
     }
     return PRRTC_OK;
 }
@@ -996,8 +1081,9 @@ int prrtc_get_tuning(int robot, prrtc_tuning *t) {
         set_error("prrtc_get_tuning: invalid argument");
         return PRRTC_EINVAL;
     }
-    t->wg_per_cu = plan_wg_per_cu(robot);
+    t->waves_per_cu = plan_waves_per_cu(robot);
     t->waves_per_task = plan_task_waves(robot);
+    t->batch_waves_per_task = plan_batch_waves(robot);
     return PRRTC_OK;
 }
 

@@ -67,21 +67,36 @@ struct PlanState {
 
 // Launch-wide control block.
 struct PlanControl {
-    int finished;              // problems that stopped running during this launch
     int n_problems;            // table size (running or not)
     int path_rows;             // rows used in the path output
     int path_cap;              // rows available in the path output
     int max_nnc;               // largest num_new_configs of the launch
-    int n_running;             // problems running when the launch started
     int mask_words;            // ceil(n_problems / 64)
-    int pad;
+    int min_running;           // the launch ends once at most this many problems run
+    int prio_policy;           // wave priority of a task: kPrioNone / kPrioOldest / kPrioYoungest
+    int pad1;
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
     unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit
+    // the running problem furthest along, (round << 32) | p, or 0: its open
+    // slots are claimed first and its tasks run at raised wave priority, so
+    // the longest problem of a batch is not slowed by the crowd
+    unsigned long long hot_key;
 };
 
 // slot outcome flags in meta[1]
 constexpr int kSkip = 1, kExtOk = 2, kConnected = 4;
+
+// Task wave priority (s_setprio) from the problem's age in rounds: a batch
+// favours its oldest problems (they bound the batch's makespan), a server
+// its youngest (they bound request latency).
+constexpr int kPrioNone = 0, kPrioOldest = 1, kPrioYoungest = 2;
+__device__ __forceinline__ int task_priority(int policy, int round, bool hot) {
+    const int age = round < 8 ? 0 : round < 64 ? 1 : round < 512 ? 2 : 3;
+    if (policy == kPrioOldest) return hot ? 3 : (age < 2 ? age : 2);
+    if (policy == kPrioYoungest) return 3 - age;
+    return 0;
+}
 
 // A planner task runs on one workgroup of 1..kMaxTaskWaves waves.
 constexpr int kMaxTaskWaves = 4;
@@ -177,16 +192,19 @@ __host__ __device__ inline void halton_bases(uint64_t seed, int slot, int dof, f
 }
 
 // First problem at or after `from` (cyclically) whose bit is set in the
-// running mask, or -1 if none: one load per lane covers 64 mask words.
-__device__ __forceinline__ int next_running(const unsigned long long *mask, int words, int from) {
+// running mask, or -1 if none; `count` receives the number of set bits.
+// One load per lane covers 64 mask words.  Wave-uniform results.
+__device__ __forceinline__ int next_running(const unsigned long long *mask, int words, int from, int &count) {
     const int lane = lane_id();
     const int w0 = from >> 6;
     const unsigned long long hi_part = ~0ull << (from & 63), lo_part = ~hi_part;
     int first_a = -1, first_b = -1;   // a: at or after `from`; b: before it (wrap-around)
+    int bits = 0;
     for (int base = 0; base < words; base += kWave) {
         const int w = base + lane;
         const unsigned long long m =
             w < words ? __hip_atomic_load(mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        bits += __builtin_popcountll(m);
         const unsigned long long a = w > w0 ? m : (w == w0 ? m & hi_part : 0ull);
         const unsigned long long b = w < w0 ? m : (w == w0 ? m & lo_part : 0ull);
         const uint64_t ba = ballot(a != 0ull), bb = ballot(b != 0ull);
@@ -196,14 +214,16 @@ __device__ __forceinline__ int next_running(const unsigned long long *mask, int 
             const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), j);
             first_b = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
         }
-        if (ba) {
+        if (first_a < 0 && ba) {
             const int j = __builtin_ctzll(ba);
             const unsigned lo = __builtin_amdgcn_readlane((unsigned)a, j);
             const unsigned hi = __builtin_amdgcn_readlane((unsigned)(a >> 32), j);
             first_a = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
-            break;
         }
     }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) bits += __shfl_xor(bits, off);
+    count = __builtin_amdgcn_readfirstlane(bits);
     return first_a >= 0 ? first_a : first_b;
 }
 

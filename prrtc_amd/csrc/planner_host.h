@@ -40,15 +40,22 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
 // Fills a PlanDesc whose buffers live in the device block at `dev`.
 void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev);
 
-// Launch shape of a robot's persistent planner grid: the value set by
+// Launch shapes of a robot's persistent planner grids: the value set by
 // prrtc_set_tuning, else an environment variable, else the default.
-//   plan_task_waves  waves per task workgroup, 1..4 (PRRTC_TASK_WAVES)
-//   plan_wg_per_cu   workgroups per CU, 1..32 (PRRTC_WG_PER_CU; default
-//                    8 / waves)
-constexpr int kMaxWgPerCu = 32;
-constexpr int kDefaultTaskWaves = 2;
+//   plan_task_waves    waves per task, latency shape: single problems, the
+//                      server, the last problems of a batch (PRRTC_TASK_WAVES)
+//   plan_batch_waves   waves per task, throughput shape: a batch while many
+//                      problems run (PRRTC_BATCH_WAVES)
+//   plan_waves_per_cu  waves per CU of a grid (PRRTC_WAVES_PER_CU)
+//   plan_grid          task workgroups of a grid with `waves` per task
+constexpr int kMaxWavesPerCu = 32;
+constexpr int kDefaultTaskWaves = 4;
+constexpr int kDefaultBatchWaves = 4;
+constexpr int kDefaultWavesPerCu = 8;
 int plan_task_waves(int robot);
-int plan_wg_per_cu(int robot);
+int plan_batch_waves(int robot);
+int plan_waves_per_cu(int robot);
+int plan_grid(int robot, int waves, long long slots);
 
 // Tree roots and Halton states of the problems listed in d_list (table
 // entries), or of entries 0..n-1 when d_list is null.
@@ -57,7 +64,7 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
 // One persistent launch over the problem table (runs until every running
 // problem stops, or until the control block's time slice ends).
 void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
-                       int grid, int lds_scene_f4, int max_ext, hipStream_t s);
+                       int grid, int waves, int lds_scene_f4, int max_ext, hipStream_t s);
 
 // Rows of one solution: goal -> start (pRRTC.cu:715-746) from the two traces
 // the commit wrote at state.path_off.  Returns the row count, or -1 if the

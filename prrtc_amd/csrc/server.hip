@@ -291,17 +291,22 @@ void ServerImpl::launch(RobotClass &c) {
     PlanControl &hc = *c.h_ctrl_up;
     hc = PlanControl{};
     hc.n_problems = c.high;
-    hc.n_running = c.running;
     hc.path_cap = c.path_rows;
     hc.max_nnc = max_nnc;
     hc.slice_ticks = (unsigned long long)cfg.slice_us * ticks_per_us;
     hc.mask_words = words;
+    hc.prio_policy = kPrioYoungest;
     hc.run_mask = c.d_mask;
     PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * words, hipMemcpyHostToDevice, s));
     PRRTC_HIP(hipMemcpyAsync(c.d_ctrl, c.h_ctrl_up, sizeof(PlanControl), hipMemcpyHostToDevice, s));
-    const int grid = (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * plan_wg_per_cu(c.robot)));
+    // throughput shape while many problems are resident, latency shape
+    // (obstacles split over the task's waves) for the last few
+    const int w_lat = plan_task_waves(c.robot), w_thr = plan_batch_waves(c.robot);
+    const int tail = std::max(1, plan_grid(c.robot, w_lat, slots) / std::max(1, max_nnc));
+    const int waves = c.running > tail ? w_thr : w_lat;
+    const int grid = plan_grid(c.robot, waves, slots);
     PRRTC_HIP(hipEventRecord(c.e0, s));
-    launch_plan_slice(c.robot, c.d_desc, c.d_state, c.d_ctrl, c.d_paths, grid, lds_f4, ext, s);
+    launch_plan_slice(c.robot, c.d_desc, c.d_state, c.d_ctrl, c.d_paths, grid, waves, lds_f4, ext, s);
     PRRTC_HIP(hipEventRecord(c.e1, s));
     PRRTC_HIP(hipMemcpyAsync(c.h_state, c.d_state, sizeof(PlanState) * (size_t)c.high, hipMemcpyDeviceToHost, s));
     PRRTC_HIP(hipMemcpyAsync(c.h_ctrl_down, c.d_ctrl, sizeof(PlanControl), hipMemcpyDeviceToHost, s));

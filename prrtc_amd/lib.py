@@ -66,7 +66,7 @@ class CBatchStats(C.Structure):
 
 
 class CTuning(C.Structure):
-    _fields_ = [("wg_per_cu", C.c_int32), ("waves_per_task", C.c_int32)]
+    _fields_ = [("waves_per_cu", C.c_int32), ("waves_per_task", C.c_int32), ("batch_waves_per_task", C.c_int32)]
 
 
 class CServerConfig(C.Structure):
@@ -102,6 +102,8 @@ SIGNATURES = [
     ("prrtc_validate_edges", C.c_int, [C.c_int, C.POINTER(CScene), C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("prrtc_validate_edges_async", C.c_int,
      [C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("prrtc_nearest_async", C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     ("prrtc_solve", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(CScene), C.POINTER(CSettings),
                               C.POINTER(CResult), C.c_void_p, C.c_int]),
     ("prrtc_solve_batch", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
@@ -184,17 +186,17 @@ def settings_default() -> dict:
     return {name: getattr(st, name) for name, _ in CSettings._fields_}
 
 
-def set_tuning(robot, wg_per_cu: int = 0, waves_per_task: int = 0) -> None:
-    """Launch shape of the persistent planner for `robot` (every robot when
-    robot is None): task workgroups per CU and waves per task; 0 = default."""
-    t = CTuning(int(wg_per_cu), int(waves_per_task))
+def set_tuning(robot, waves_per_cu: int = 0, waves_per_task: int = 0, batch_waves_per_task: int = 0) -> None:
+    """Launch shapes of the persistent planner for `robot` (every robot when
+    robot is None); 0 = default.  See prrtc_tuning in include/prrtc.h."""
+    t = CTuning(int(waves_per_cu), int(waves_per_task), int(batch_waves_per_task))
     _check(load().prrtc_set_tuning(-1 if robot is None else robot_id(robot), C.byref(t)), "prrtc_set_tuning")
 
 
 def get_tuning(robot) -> dict:
     t = CTuning()
     _check(load().prrtc_get_tuning(robot_id(robot), C.byref(t)), "prrtc_get_tuning")
-    return {"wg_per_cu": t.wg_per_cu, "waves_per_task": t.waves_per_task}
+    return {name: getattr(t, name) for name, _ in CTuning._fields_}
 
 
 def pack_scene(scene: Scene) -> np.ndarray:
@@ -239,6 +241,13 @@ def validate_edges_async(robot, d_scene: int, scene_floats: int, d_edges: int, n
     """Device-pointer form: all pointers are device addresses (ints)."""
     _check(load().prrtc_validate_edges_async(robot_id(robot), d_scene, scene_floats, d_edges, n, granularity,
                                              d_out, stream), "prrtc_validate_edges_async")
+
+
+def nearest_async(dof: int, d_trees: int, cap: int, n_nodes: int, n_trees: int, d_queries: int,
+                  queries_per_tree: int, waves: int, d_idx: int, d_dist2: int, stream: int = 0) -> None:
+    """Nearest-node search on device buffers (all pointers device addresses)."""
+    _check(load().prrtc_nearest_async(dof, d_trees, cap, n_nodes, n_trees, d_queries, queries_per_tree, waves,
+                                      d_idx, d_dist2, stream), "prrtc_nearest_async")
 
 
 def _result_dict(r: CResult) -> dict:

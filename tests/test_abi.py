@@ -71,3 +71,22 @@ def test_invalid_arguments_fail_without_gpu():
     assert rc == -1
     assert b"invalid" in L.prrtc_last_error()
     assert L.prrtc_validate_edges(9, C.byref(CScene()), None, 0, 32, None) == -1
+    # nearest-node search: cap must be whole 1024-node chunks, dof one of the robots'
+    assert L.prrtc_nearest_async(7, 1, 1000, 10, 1, 1, 1, 1, 1, 1, None) == -1
+    assert L.prrtc_nearest_async(5, 1, 1024, 10, 1, 1, 1, 1, 1, 1, None) == -1
+    assert L.prrtc_nearest_async(7, 1, 1024, 2000, 1, 1, 1, 1, 1, 1, None) == -1
+    assert L.prrtc_nearest_async(7, 1, 1024, 10, 1, 1, 1, 9, 1, 1, None) == -1
+    # tuning ranges
+    from prrtc_amd.lib import CTuning
+    assert L.prrtc_set_tuning(1, C.byref(CTuning(0, 9, 0))) == -1
+    assert L.prrtc_set_tuning(7, C.byref(CTuning(0, 0, 0))) == -1
+
+
+def test_server_rejects_bad_config_without_gpu():
+    from prrtc_amd.lib import CServerConfig, load
+    L = load()
+    h = C.c_void_p()
+    cfg = CServerConfig(0, -5, 0, 0, 0)      # negative table size
+    assert L.prrtc_server_create(C.byref(cfg), C.byref(h)) == -1
+    assert not h.value
+    assert L.prrtc_server_create(None, None) == -1

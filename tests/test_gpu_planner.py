@@ -175,12 +175,18 @@ def test_forced_failures(gpu, oracle_robots):
 
 @pytest.mark.parametrize("waves", [2, 4])
 def test_task_waves_same_results(waves, gpu, oracle_robots):
-    """A task spread over several waves (shared nearest-node scans, edge and
-    chain poses over all lanes) returns bitwise the single-wave result."""
+    """A task spread over several waves (shared nearest-node scans, obstacle
+    tables split between the waves of each validation pass) returns bitwise
+    the single-wave result, alone and in a batch that switches from the
+    one-wave throughput grid to the multi-wave latency grid for its last
+    problems."""
     from prrtc_amd import scenes as S
     from prrtc_amd.robots import fixed_spheres, limits
     cases = []
-    for i, robot in enumerate(["sphere", "panda", "fetch", "baxter"]):
+    # 12 point-robot problems make their group large enough (12 x 48 slots
+    # over a 512-task latency grid) for the batch to run two launch shapes
+    for i in range(21):
+        robot = ["sphere", "panda", "fetch", "baxter"][i % 4] if i < 12 else "sphere"
         R = oracle_robots[robot]
         if robot == "sphere":
             sc = point_scene(10 + i)
@@ -189,19 +195,19 @@ def test_task_waves_same_results(waves, gpu, oracle_robots):
             sc = S.scene_mixed(300 + i, 8, 24, avoid=fixed_spheres(robot))
             lo, span = limits(robot)
         start, goal = free_pair(R, sc, lo, span, np.random.default_rng(70 + i), far=True)
-        st = dict(num_new_configs=48, granularity=[16, 64, 128, 100][i], range=0.5, balance=i % 3,
+        st = dict(num_new_configs=48, granularity=[16, 64, 128, 100][i % 4], range=0.5, balance=i % 3,
                   tree_ratio=0.5, dynamic_domain=i % 2, max_samples=30000, max_iters=120, seed=i)
         cases.append((robot, start, goal[None], sc, st))
     try:
-        gpu.set_tuning(None, waves_per_task=1)
+        gpu.set_tuning(None, waves_per_task=1, batch_waves_per_task=1)
         ref = [gpu.solve(*c) for c in cases]
-        gpu.set_tuning(None, waves_per_task=waves)
+        gpu.set_tuning(None, waves_per_task=waves, batch_waves_per_task=1)
         assert gpu.get_tuning("panda")["waves_per_task"] == waves
         got = [gpu.solve(*c) for c in cases]
         batch, bpaths, _ = gpu.solve_batch([dict(robot=c[0], start=c[1], goals=c[2], scene=c[3]) for c in cases],
                                            [c[4] for c in cases])
     finally:
-        gpu.set_tuning(None, 0, 0)
+        gpu.set_tuning(None, 0, 0, 0)
     for (robot, *_), (r1, p1), (r2, p2), rb, pb in zip(cases, ref, got, batch, bpaths):
         print(robot, r1["solved"], r1["iters"], r1["start_tree_size"], r1["goal_tree_size"], flush=True)
         for k in ("solved", "iters", "start_tree_size", "goal_tree_size", "path_length", "edge_checks"):
