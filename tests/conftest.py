@@ -36,7 +36,13 @@ def oracle_robots():
 
 @pytest.fixture(scope="session")
 def gpu():
-    """The HIP library, loaded; skips nothing -- a missing library is an error."""
+    """The HIP library, loaded; skips nothing -- a missing library is an error.
+
+    torch (used by the device-buffer tests) brings its own HIP runtime, which
+    must see the GPU before the library's runtime initialises it."""
+    import torch
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda:0")
     import prrtc_amd
     prrtc_amd.load()
     return prrtc_amd
