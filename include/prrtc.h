@@ -161,14 +161,15 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
  * per problem, src/planning/pRRTC.cu:700-702).  0 = default for each field.
  *   waves_per_cu          waves per CU of a persistent grid, 1..32
  *                         (environment PRRTC_WAVES_PER_CU, else 8)
- *   waves_per_task        waves (64 lanes each) sharing one planner task in
- *                         the latency shape -- single solves, a server with
- *                         few resident problems, the last problems of a
- *                         batch: they share the nearest-node scans and split
- *                         the obstacle tables of every validation pass
- *                         (1..4; environment PRRTC_TASK_WAVES, else 4)
- *   batch_waves_per_task  the same for the throughput shape, while many
- *                         problems run (1..4; PRRTC_BATCH_WAVES, else 4)
+ *   waves_per_task        waves (64 lanes each) sharing one planner task of a
+ *                         single solve, or of a server slice with few
+ *                         resident problems: they share the nearest-node
+ *                         scans and split the obstacle tables of every
+ *                         validation pass (1..4; environment
+ *                         PRRTC_TASK_WAVES, else 4)
+ *   batch_waves_per_task  the same for prrtc_solve_batch and for server
+ *                         slices with many resident problems (1..4;
+ *                         PRRTC_BATCH_WAVES, else 4)
  * robot = -1 sets every robot.  Applies to later prrtc_solve,
  * prrtc_solve_batch and server slices; results do not depend on it. */
 typedef struct prrtc_tuning {

@@ -8,9 +8,10 @@ obstacles, g 128, dynamic domain), keeps the problems a pilot run connects,
 and measures on the GPU:
 
 * batched form (one persistent launch): plans/s over num_new_configs x waves
-  per task x waves per CU; the best sets ``batch_nnc`` and the launch shape;
+  per task x waves per CU; the best sets ``batch_nnc``, ``waves_per_cu`` and
+  ``batch_waves_per_task``;
 * plain form (one problem per launch): plans/s over num_new_configs x waves
-  per task on a subset; the best sets ``plain_nnc`` and the latency shape.
+  per task on a subset; the best sets ``plain_nnc`` and ``waves_per_task``.
 
 A lane always carries one pose (64 poses per wavefront); the tuned shape is
 how many wavefronts share one task (they split the obstacle tables of each
@@ -87,7 +88,7 @@ def tune_robot(P, robot, n, pilot_iters, max_iters, plain_n):
     sub = feas[:plain_n]
     best_p = None
     for waves in (2, 4):
-        P.set_tuning(robot, waves_per_cu=best_b[2], waves_per_task=waves, batch_waves_per_task=best_b[1])
+        P.set_tuning(robot, waves_per_cu=best_b[2], waves_per_task=waves, batch_waves_per_task=waves)
         for nnc in (64, 128, 256, 512):
             st = dict(base, num_new_configs=nnc, max_iters=max_iters)
             dt, res = timed(lambda: [P.solve(p["robot"], p["start"], p["goals"], p["scene"], st) for p in sub])

@@ -425,7 +425,6 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
     int staged = -1;
     int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
-    const int min_running = ctrl->min_running;
     const int prio_policy = ctrl->prio_policy;
     const int max_nnc = ctrl->max_nnc;
     const unsigned long long slice = ctrl->slice_ticks;
@@ -467,10 +466,8 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor, running));
                 // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
-                // share their counters with the whole grid.  A launch with
-                // min_running > 0 ends once that few problems are left (the
-                // next launch, shaped for latency, finishes them).
-                if (running <= min_running || (long long)blockIdx.x >= (long long)running * max_nnc) break;
+                // share their counters with the whole grid
+                if (running == 0 || (long long)blockIdx.x >= (long long)running * max_nnc) break;
                 // a time-sliced launch ends between tasks; unfinished rounds
                 // resume in the next launch from the state kept in device memory
                 if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
@@ -822,6 +819,11 @@ int plan_waves_per_cu(int robot) {
     return tuned(g_waves_per_cu, robot, env, kDefaultWavesPerCu);
 }
 
+int plan_tail_problems(int robot, int max_nnc) {
+    // the throughput grid holds every open task of this many problems at once
+    return std::max(1, num_cus() * plan_waves_per_cu(robot) / std::max(1, max_nnc));
+}
+
 int plan_grid(int robot, int waves, long long slots) {
     const long long per_cu = std::max(1, plan_waves_per_cu(robot) / waves);
     return (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * per_cu));
@@ -893,17 +895,12 @@ static void start_group(GroupRun &gr, int path_cap) {
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
     for (int i = 0; i < P; i++)
         PRRTC_HIP(hipMemcpyAsync(dev + o_block[i], host + h_block[i], lay[i].upload_bytes, hipMemcpyHostToDevice, s));
-    // --- launch shapes: a throughput grid (batch_waves per task) while many
-    // problems run, then a latency grid (task waves, obstacles split between
-    // them) for the last few; both launches are queued back to back and the
-    // first leaves once at most `tail` problems run
+    // --- launch shape: a single problem runs its tasks on waves_per_task
+    // waves (latency), a batch on batch_waves_per_task (throughput); one
+    // persistent grid either way
     long long total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
-    const int w_lat = plan_task_waves(robot), w_thr = plan_batch_waves(robot);
-    const int grid_lat = plan_grid(robot, w_lat, total_slots);
-    const int tail = std::max(1, grid_lat / std::max(1, hc->max_nnc));
-    const bool two_phase = w_thr != w_lat && P > tail;
-    hc->min_running = two_phase ? tail : 0;
+    const int waves = P == 1 ? plan_task_waves(robot) : plan_batch_waves(robot);
     hc->prio_policy = kPrioOldest;
     const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
@@ -911,12 +908,8 @@ static void start_group(GroupRun &gr, int path_cap) {
     float *d_paths = reinterpret_cast<float *>(dev + o_paths);
     PRRTC_HIP(hipEventRecord(ws.e0, s));
     launch_plan_init(robot, d_desc, d_state, nullptr, P, s);
-    if (two_phase) {
-        launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, w_thr, total_slots), w_thr,
-                          lds_f4, ext_all, s);
-        PRRTC_HIP(hipMemsetAsync(&d_ctrl->min_running, 0, sizeof(int), s));
-    }
-    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid_lat, w_lat, lds_f4, ext_all, s);
+    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, waves, total_slots), waves, lds_f4,
+                      ext_all, s);
     PRRTC_HIP(hipEventRecord(ws.e1, s));
     // --- queue the readback of states + control
     PRRTC_HIP(hipMemcpyAsync(host + h_read, dev + o_state, sizeof(PlanState) * P, hipMemcpyDeviceToHost, s));

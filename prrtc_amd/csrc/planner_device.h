@@ -4,15 +4,16 @@
 // looping sample -> nearest -> steer -> validate -> insert -> connect with
 // the trees shared through global atomics (src/planning/pRRTC.cu:305-602).
 // Here the same per-sample step is a *task* (problem p, slot s) executed by
-// one wave64, and a problem advances in rounds: every slot of a round works
-// against the trees as committed at the round start; the wave that finishes
-// the round's last task commits the new nodes in slot order, applies the
-// dynamic-domain radius updates in slot order, and detects the solution.
-// Results are therefore a deterministic function of (problem, settings,
-// seed), identical whether a problem runs alone or among hundreds.
+// one workgroup of 1-4 wave64s, and a problem advances in rounds: every slot
+// of a round works against the trees as committed at the round start; the
+// workgroup that finishes the round's last task commits the new nodes in
+// slot order, applies the dynamic-domain radius updates in slot order, and
+// detects the solution.  Results are therefore a deterministic function of
+// (problem, settings, seed), identical whether a problem runs alone or among
+// hundreds, and whatever the launch shape.
 //
-// Persistent kernel: a fixed grid of single-wave workgroups pulls tasks of
-// any running problem from per-problem slot counters; problems of one launch
+// Persistent kernel: a fixed grid of task workgroups pulls tasks of any
+// running problem from per-problem slot counters; problems of one launch
 // have independent trees, scenes and settings.
 #pragma once
 
@@ -72,9 +73,8 @@ struct PlanControl {
     int path_cap;              // rows available in the path output
     int max_nnc;               // largest num_new_configs of the launch
     int mask_words;            // ceil(n_problems / 64)
-    int min_running;           // the launch ends once at most this many problems run
     int prio_policy;           // wave priority of a task: kPrioNone / kPrioOldest / kPrioYoungest
-    int pad1;
+    int pad0, pad1;
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
     unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit

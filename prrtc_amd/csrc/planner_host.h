@@ -42,10 +42,10 @@ void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *d
 
 // Launch shapes of a robot's persistent planner grids: the value set by
 // prrtc_set_tuning, else an environment variable, else the default.
-//   plan_task_waves    waves per task, latency shape: single problems, the
-//                      server, the last problems of a batch (PRRTC_TASK_WAVES)
-//   plan_batch_waves   waves per task, throughput shape: a batch while many
-//                      problems run (PRRTC_BATCH_WAVES)
+//   plan_task_waves    waves per task of a single solve, or of a server
+//                      slice with few resident problems (PRRTC_TASK_WAVES)
+//   plan_batch_waves   waves per task of a batch, or of a server slice with
+//                      many resident problems (PRRTC_BATCH_WAVES)
 //   plan_waves_per_cu  waves per CU of a grid (PRRTC_WAVES_PER_CU)
 //   plan_grid          task workgroups of a grid with `waves` per task
 constexpr int kMaxWavesPerCu = 32;
@@ -56,6 +56,9 @@ int plan_task_waves(int robot);
 int plan_batch_waves(int robot);
 int plan_waves_per_cu(int robot);
 int plan_grid(int robot, int waves, long long slots);
+// Resident problems up to which a server slice runs the single-solve shape:
+// as many as a one-wave grid holds with all their tasks in flight.
+int plan_tail_problems(int robot, int max_nnc);
 
 // Tree roots and Halton states of the problems listed in d_list (table
 // entries), or of entries 0..n-1 when d_list is null.

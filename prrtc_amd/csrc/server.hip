@@ -302,7 +302,7 @@ void ServerImpl::launch(RobotClass &c) {
     // throughput shape while many problems are resident, latency shape
     // (obstacles split over the task's waves) for the last few
     const int w_lat = plan_task_waves(c.robot), w_thr = plan_batch_waves(c.robot);
-    const int tail = std::max(1, plan_grid(c.robot, w_lat, slots) / std::max(1, max_nnc));
+    const int tail = plan_tail_problems(c.robot, max_nnc);
     const int waves = c.running > tail ? w_thr : w_lat;
     const int grid = plan_grid(c.robot, waves, slots);
     PRRTC_HIP(hipEventRecord(c.e0, s));
