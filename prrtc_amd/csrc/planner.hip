@@ -471,9 +471,11 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // a time-sliced launch ends between tasks; unfinished rounds
                 // resume in the next launch from the state kept in device memory
                 if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
-                // the problem furthest along first
-                const unsigned long long hk = __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT);
+                // a batch serves the problem furthest along first
+                const unsigned long long hk =
+                    prio_policy == kPrioOldest
+                        ? __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0ull;
                 const int ph = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
                 if (ph >= 0 && ph < P) {
                     const int s = try_claim(ph);
