@@ -155,13 +155,25 @@ def main():
     roof_ms = time_launches(roof_step, args.steps, torch)
     assert bool(d_roof_out.all()), "free-edge set must validate free"
     if args.profile_run:
+        # the streaming nearest-node case of the bench (one query per tree)
+        n_trees, n_nodes, ndof = 256, 100_000, 8
+        ncap = ((n_nodes + 1023) // 1024) * 1024
+        gen = torch.Generator(device=dev).manual_seed(3)
+        d_trees = torch.rand((n_trees, ndof, ncap), generator=gen, device=dev)
+        d_q = torch.rand((n_trees, 1, ndof), generator=gen, device=dev)
+        d_idx = torch.zeros(n_trees, dtype=torch.int32, device=dev)
+        d_d2 = torch.zeros(n_trees, dtype=torch.float32, device=dev)
+        nn_ms = time_launches(lambda: P.nearest_async(ndof, d_trees.data_ptr(), ncap, n_nodes, n_trees,
+                                                      d_q.data_ptr(), 1, 4, d_idx.data_ptr(), d_d2.data_ptr(),
+                                                      stream), args.steps, torch)
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)       # 256 MiB
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"profile_run": True, "validate_launches": args.steps + 1, "edges_per_launch": n_roof,
-                          "kernel_ms": roof_ms, "copy_bytes_read": src.numel() * 4,
-                          "copy_bytes_written": dst.numel() * 4}), flush=True)
+                          "kernel_ms": roof_ms, "nearest_launches": args.steps, "nearest_ms": nn_ms,
+                          "nearest_bytes_per_launch": 4 * ndof * n_nodes * n_trees,
+                          "copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}), flush=True)
         return
     rng = np.random.default_rng(5)
     pick = roof_edges[rng.integers(0, n_roof, 512)]
