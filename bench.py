@@ -436,6 +436,26 @@ def main():
                                          "p99_ms": pct(times, 99), "cores": 1,
                                          "kind": "project CPU RRT-Connect (vamp unavailable offline)"}
             log(f"cpu rrt-connect c0: p50 {out['cpu_rrt_connect_c0']['p50_ms']:.1f} ms")
+        if "batched" in out:
+            # the same connectable Fetch problems through the CPU RRT-Connect,
+            # one problem per host thread (ctypes releases the GIL), bounded
+            # at 5000 samples per problem
+            from concurrent.futures import ThreadPoolExecutor
+            fprobs = [p for p, r in zip(bprobs, pres) if r["solved"]]
+            fr = model("fetch")
+
+            def cpu_solve(p):
+                return fr.rrt_connect(p["scene"], p["start"], p["goals"][0], 64, 0.5, 5000, 1)[1].solved
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                cpu_solved = sum(ex.map(cpu_solve, fprobs))
+            dt = time.perf_counter() - t0
+            out["cpu_rrt_connect_c2"] = {"problems": len(fprobs), "solved": int(cpu_solved),
+                                         "plans_per_s": cpu_solved / dt, "wall_s": dt, "cores": threads,
+                                         "kind": "project CPU RRT-Connect (vamp unavailable offline)",
+                                         "settings": "fetch, g 64, range 0.5, max 5000 samples per problem"}
+            log(f"cpu rrt-connect c2: {cpu_solved}/{len(fprobs)} solved in {dt:.1f} s on {threads} threads")
         bad = 0
         for robot, sc, path, gg in paths_to_check:
             bad += model(robot).revalidate(sc, path, gg) != -1

@@ -126,24 +126,33 @@ template <int N, bool EARLY>
 __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
                                           const float (&z)[N], const float (&r)[N], bool (&hit)[N]) {
 #pragma clang fp contract(off)
+    // Each sphere keeps the smallest test value over the obstacles seen: some
+    // value < 0 (a collision) iff the minimum is < 0.  One v_min per test
+    // replaces a compare, a select and the bit packing of a boolean OR; a
+    // NaN test value is dropped by fminf as it would fail `< 0`.
+    float m[N];
 #pragma unroll
-    for (int n = 0; n < N; n++) hit[n] = false;
+    for (int n = 0; n < N; n++) m[n] = __builtin_huge_valf();
     auto done = [&]() {
         bool any_hit = false;
 #pragma unroll
-        for (int n = 0; n < N; n++) any_hit |= hit[n];
+        for (int n = 0; n < N; n++) any_hit |= m[n] < 0.0f;
         return ballot(!any_hit) == 0;
+    };
+    auto finish = [&]() {
+#pragma unroll
+        for (int n = 0; n < N; n++) hit[n] = m[n] < 0.0f;
     };
     for (int i = 0; i < o.n_sph; i++) {                       // utils.cuh:58-81
         const float4 s = o.sph[i];
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float rs = s.w + r[n];
-            hit[n] |= fmaf(-rs, rs, sq3(s.x - x[n], s.y - y[n], s.z - z[n])) < 0.0f;
+            m[n] = fminf(m[n], fmaf(-rs, rs, sq3(s.x - x[n], s.y - y[n], s.z - z[n])));
         }
-        if (EARLY && (i & 7) == 7 && done()) return;
+        if (EARLY && (i & 7) == 7 && done()) return finish();
     }
-    if (EARLY && o.n_sph && done()) return;
+    if (EARLY && o.n_sph && done()) return finish();
     for (int i = 0; i < o.n_cap; i++) {                       // utils.cuh:113-126
         const float4 p = o.cap[2 * i], v = o.cap[2 * i + 1];
 #pragma unroll
@@ -152,11 +161,11 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
             const float rs = r[n] + p.w;
             const float d = sq3(x[n] - fmaf(v.x, cdf, p.x), y[n] - fmaf(v.y, cdf, p.y), z[n] - fmaf(v.z, cdf, p.z));
-            hit[n] |= fmaf(-rs, rs, d) < 0.0f;
+            m[n] = fminf(m[n], fmaf(-rs, rs, d));
         }
-        if (EARLY && (i & 7) == 7 && done()) return;
+        if (EARLY && (i & 7) == 7 && done()) return finish();
     }
-    if (EARLY && o.n_cap && done()) return;
+    if (EARLY && o.n_cap && done()) return finish();
     for (int i = 0; i < o.n_zcap; i++) {                      // utils.cuh:135-148
         const float4 p = o.zcap[2 * i], v = o.zcap[2 * i + 1];
 #pragma unroll
@@ -165,11 +174,11 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
             const float rs = r[n] + p.w;
             const float d = sq3(x[n] - p.x, y[n] - p.y, z[n] - fmaf(v.z, cdf, p.z));
-            hit[n] |= fmaf(-rs, rs, d) < 0.0f;
+            m[n] = fminf(m[n], fmaf(-rs, rs, d));
         }
-        if (EARLY && (i & 7) == 7 && done()) return;
+        if (EARLY && (i & 7) == 7 && done()) return finish();
     }
-    if (EARLY && o.n_zcap && done()) return;
+    if (EARLY && o.n_zcap && done()) return finish();
     for (int i = 0; i < o.n_cub; i++) {                       // utils.cuh:158-175
         const float4 c = o.cub[4 * i], a1 = o.cub[4 * i + 1], a2 = o.cub[4 * i + 2], a3 = o.cub[4 * i + 3];
 #pragma unroll
@@ -178,11 +187,11 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, fmaf(a1.y, ys, a1.z * zs))) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, fmaf(a2.y, ys, a2.z * zs))) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(fmaf(a3.x, xs, fmaf(a3.y, ys, a3.z * zs))) - a2.w);
-            hit[n] |= fmaf(-r[n], r[n], sq3(d1, d2, d3)) < 0.0f;
+            m[n] = fminf(m[n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
         }
-        if (EARLY && (i & 7) == 7 && done()) return;
+        if (EARLY && (i & 7) == 7 && done()) return finish();
     }
-    if (EARLY && o.n_cub && done()) return;
+    if (EARLY && o.n_cub && done()) return finish();
     for (int i = 0; i < o.n_zcub; i++) {                      // utils.cuh:184-201
         const float4 c = o.zcub[4 * i], a1 = o.zcub[4 * i + 1], a2 = o.zcub[4 * i + 2];
 #pragma unroll
@@ -191,10 +200,11 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, a1.y * ys)) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, a2.y * ys)) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(zs) - a2.w);
-            hit[n] |= fmaf(-r[n], r[n], sq3(d1, d2, d3)) < 0.0f;
+            m[n] = fminf(m[n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
         }
-        if (EARLY && (i & 7) == 7 && done()) return;
+        if (EARLY && (i & 7) == 7 && done()) return finish();
     }
+    finish();
 }
 
 // sphere_sphere_self_collision (utils.cuh:83-86) with the radius sum folded.
