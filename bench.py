@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-young", type=int, default=0,
+                    help="server young_rounds: rounds a request is served ahead of older ones (0: default 64, "
+                         "-1: round-robin)")
     ap.add_argument("--cpu-edges", type=int, default=500_000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plans", action="store_true")
@@ -342,8 +345,9 @@ def main():
         nb, npd, nf = (max(1, int(round(k * args.mix_scale))) for k in (128, 256, 128))
         log(f"uneven mix: {nb} Baxter + {npd} Panda + {nf} Fetch")
         mprobs = workloads.c3_problems(nb, npd, nf, seed=0)
-        mnnc = tuned.get("batch_nnc", {})
-        mst = [dict(num_new_configs=mnnc.get(p["robot"], 128), granularity=128, range=0.5, balance=1, tree_ratio=0.5,
+        # nnc 128 for every class: a problem that ends at max_iters costs
+        # max_iters x nnc tasks, and the mix has such problems
+        mst = [dict(num_new_configs=128, granularity=128, range=0.5, balance=1, tree_ratio=0.5,
                     dynamic_domain=1, dd_radius=4.0, dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000,
                     max_iters=3000, seed=1) for p in mprobs]
         t0 = time.perf_counter()
@@ -357,7 +361,7 @@ def main():
             d["solved"] += r["solved"]
         out["mix"] = {"problems": len(mprobs), "solved": n_m, "plans_per_s": n_m / t_mix, "wall_s": t_mix,
                       "kernel_ms": mstats["kernel_ns"] / 1e6, "per_robot": per_robot,
-                      "settings": "g 128, balance 1, ratio 0.5, dd on, range 0.5, max_iters 3000; one persistent "
+                      "settings": "nnc 128, g 128, balance 1, ratio 0.5, dd on, range 0.5, max_iters 3000; one persistent "
                                   "grid per robot class, all classes concurrently"}
         log(f"mix: {n_m}/{len(mprobs)} solved in {t_mix:.3f} s ({n_m / t_mix:.1f} plans/s)")
         for p, r, pa in zip(mprobs, mres, mpaths):
@@ -369,12 +373,11 @@ def main():
         n_req = max(1, int(args.serve_rate * args.serve_seconds))
         log(f"serving: {n_req} requests at {args.serve_rate:.0f}/s")
         reqs = [workloads.serving_request(i, seed=0) for i in range(n_req)]
-        snnc = tuned.get("serve_nnc", {})
-        sst = {r: dict(num_new_configs=snnc.get(r, 64), granularity=64, range=0.5, balance=2, tree_ratio=1.0,
+        sst = {r: dict(num_new_configs=64, granularity=64, range=0.5, balance=2, tree_ratio=1.0,
                        dynamic_domain=0, max_samples=50000, max_iters=3000, seed=1)
                for r in ("panda", "fetch", "baxter")}
         arrivals = np.cumsum(np.random.default_rng(11).exponential(1.0 / args.serve_rate, n_req))
-        with P.Server(budget_bytes=16 << 30, slice_us=1000) as srv:
+        with P.Server(budget_bytes=16 << 30, slice_us=1000, young_rounds=args.serve_young) as srv:
             t_start = time.perf_counter()
             tickets = []
             for a, rq in zip(arrivals, reqs):
@@ -394,7 +397,8 @@ def main():
                           "p50_ms_solved": pct(lat_solved, 50), "p99_ms_solved": pct(lat_solved, 99),
                           "admission_wait_p50_ms": pct(queued, 50), "admission_wait_p99_ms": pct(queued, 99),
                           "slices": sstats["slices"], "peak_resident_mib": sstats["peak_resident_bytes"] / 2**20,
-                          "settings": "g 64, nnc 64, range 0.5, balance 2, max_iters 3000; 1 ms slices, 16 GiB budget"}
+                          "settings": "g 64, nnc 64, range 0.5, balance 2, max_iters 3000; 1 ms slices, 16 GiB budget, "
+                                      f"young_rounds {args.serve_young or 64}"}
         log(f"serving: {n_s}/{n_req} solved, p50 {out['serving']['p50_ms']:.1f} ms, "
             f"p99 {out['serving']['p99_ms']:.1f} ms, {out['serving']['plans_per_s']:.1f} plans/s")
         for rq, (r, pa, _) in zip(reqs[:200], outs[:200]):

@@ -198,6 +198,9 @@ typedef struct prrtc_server_config {
     int32_t max_pending;      /* queued requests before submit returns EBUSY (0: 65536) */
     int32_t slice_us;         /* length of one persistent launch (0: 1000) */
     int32_t path_cap;         /* rows kept per result path (0: 1024) */
+    /* a request's slots are claimed ahead of older requests' for its first
+     * young_rounds planner rounds (0: 64; < 0: never, plain round-robin) */
+    int32_t young_rounds;
 } prrtc_server_config;
 
 typedef struct prrtc_server_stats {

@@ -21,8 +21,13 @@ candidate is timed on the same problems and the choice is by solved plans
 per second.  The result is JSON, read by bench.py:
 
     {"launch": {robot: {"waves_per_cu", "waves_per_task", "batch_waves_per_task"}},
-     "batch_nnc": {robot: n}, "plain_nnc": {robot: n}, "serve_nnc": {robot: n},
+     "batch_nnc": {robot: n}, "plain_nnc": {robot: n},
      "measured": {...every candidate's plans/s...}}
+
+The nnc choice is measured on problems a pilot run found feasible.  A
+workload with problems that end at max_iters pays max_iters x nnc tasks for
+each of them, so a larger nnc costs such workloads (a serving stream, a
+mixed batch with hard scenes) more than it gains; they keep their own nnc.
 """
 from __future__ import annotations
 
@@ -113,7 +118,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     import prrtc_amd as P
     P.load()
-    out = {"launch": {}, "batch_nnc": {}, "plain_nnc": {}, "serve_nnc": {}, "measured": {},
+    out = {"launch": {}, "batch_nnc": {}, "plain_nnc": {}, "measured": {},
            "problems": args.problems}
     for robot in args.robots.split(","):
         n = args.problems if robot != "baxter" else max(8, args.problems // 3)
@@ -121,7 +126,6 @@ def main(argv=None):
         out["launch"][robot] = launch
         out["batch_nnc"][robot] = nb
         out["plain_nnc"][robot] = npl
-        out["serve_nnc"][robot] = min(nb, npl)
         out["measured"][robot] = measured
         log(f"{robot}: launch {launch}, batch nnc {nb}, plain nnc {npl}")
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

@@ -391,10 +391,14 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
             st->round = st->round + 1;
             st->done_slots = 0;
             store_release(&st->next_slot, 0);
+            __hip_atomic_fetch_or(ctrl->open_mask + (p >> 6), 1ull << (p & 63), __ATOMIC_RELEASE,
+                                  __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_max(&ctrl->hot_key, key + (1ull << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             store_release(&st->next_slot, kClaimedOut);
             __hip_atomic_fetch_and(ctrl->run_mask + (p >> 6), ~(1ull << (p & 63)), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_and(ctrl->open_mask + (p >> 6), ~(1ull << (p & 63)), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             // a finished problem hands the priority back to the next commit
             unsigned long long expect = key;
@@ -450,6 +454,8 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             // this round.  Plain loads come first, so the counters' cache
             // lines stay cheap to share; a claim counts only below nnc and
             // finished problems are never incremented, so it cannot wrap.
+            // The claim of the last slot closes the problem in the open mask
+            // (before that task can finish, so before the commit reopens it).
             auto try_claim = [&](int q) -> int {
                 PlanState *st = states + q;
                 const int nnc = descs[q].nnc;
@@ -457,17 +463,20 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 const bool open = live && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
                                               &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
                 const int s = open ? wave_atomic_add(&st->next_slot, 1) : nnc;
+                if (s == nnc - 1 && lane_id() == 0)
+                    __hip_atomic_fetch_and(ctrl->open_mask + (q >> 6), ~(1ull << (q & 63)), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
                 return s >= 0 && s < nnc ? s : -1;
             };
             while (true) {
-                // the next running problem from the cursor on (finished ones
-                // are skipped through the mask, not visited one by one)
-                int running = 0;
-                const int p = __builtin_amdgcn_readfirstlane(next_running(ctrl->run_mask, words, cursor, running));
+                // running count and the next open problem from the cursor on
+                // (young ones first on a server); closed and finished
+                // problems are skipped through the masks, not visited
+                const MaskScan ms = scan_masks(ctrl->run_mask, ctrl->open_mask, ctrl->young_mask, words, cursor);
                 // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
                 // share their counters with the whole grid
-                if (running == 0 || (long long)blockIdx.x >= (long long)running * max_nnc) break;
+                if (ms.running == 0 || (long long)blockIdx.x >= (long long)ms.running * max_nnc) break;
                 // a time-sliced launch ends between tasks; unfinished rounds
                 // resume in the next launch from the state kept in device memory
                 if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
@@ -488,18 +497,40 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                         break;
                     }
                 }
-                const int s = try_claim(p);
+                const int q = __builtin_amdgcn_readfirstlane(ms.open_young >= 0 ? ms.open_young : ms.open);
+                if (q >= 0) {
+                    const int s = try_claim(q);
+                    if (s >= 0) {
+                        // the cursor stays: the next task is likely from the
+                        // same problem, whose scene is already staged
+                        cursor = q;
+                        p_out = q;
+                        s_out = s;
+                        idle = 0;
+                        backoff = 1;
+                        break;
+                    }
+                    // another workgroup took the last slot first: look on
+                    cursor = q + 1 < P ? q + 1 : 0;
+                    continue;
+                }
+                // every running round looks fully claimed: visit the running
+                // problems in turn (which also finds a round whose open bit
+                // was cleared late), and after a pass without work sleep
+                // until a commit reopens one (exponential back-off, up to
+                // 4 x 64 x 64 cycles)
+                const int r = __builtin_amdgcn_readfirstlane(ms.first_run);
+                const int s = r >= 0 ? try_claim(r) : -1;
                 if (s >= 0) {
-                    p_out = p;
+                    cursor = r;
+                    p_out = r;
                     s_out = s;
                     idle = 0;
                     backoff = 1;
                     break;
                 }
-                // move on; after visiting every running problem without work,
-                // sleep (exponential back-off, up to 4 x 64 x 64 cycles)
-                cursor = p + 1 < P ? p + 1 : 0;
-                if (++idle >= running) {
+                cursor = r + 1 < P ? r + 1 : 0;
+                if (++idle >= ms.running) {
                     for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
                     backoff = backoff < 4 ? 2 * backoff : 4;
                     idle = 0;
@@ -855,7 +886,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     const size_t o_state = dl.take(sizeof(PlanState) * P);
     const size_t o_ctrl = dl.take(sizeof(PlanControl));
     const int words = (P + 63) / 64;
-    const size_t o_mask = dl.take(sizeof(unsigned long long) * words);
+    const size_t o_mask = dl.take(sizeof(unsigned long long) * 2 * words);   // running | open
     std::vector<BlockLayout> lay(P);
     std::vector<size_t> o_block(P);
     int lds_f4 = 0, ext_all = 0;
@@ -871,7 +902,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     char *dev = ws.need_dev(dl.off);
     // --- host image of everything that is uploaded: descs + control, then
     // each problem's scene and init rows
-    const size_t up_bytes = o_mask + sizeof(unsigned long long) * words;
+    const size_t up_bytes = o_mask + sizeof(unsigned long long) * 2 * words;
     Align hl;
     const size_t h_hdr = hl.take(up_bytes);
     std::vector<size_t> h_block(P);
@@ -888,8 +919,9 @@ static void start_group(GroupRun &gr, int path_cap) {
     hc->n_problems = P;
     hc->mask_words = words;
     hc->run_mask = reinterpret_cast<unsigned long long *>(dev + o_mask);
+    hc->open_mask = hc->run_mask + words;   // every problem starts with round 0 open
     unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + h_hdr + o_mask);
-    for (int w = 0; w < words; w++) hm[w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
+    for (int w = 0; w < words; w++) hm[w] = hm[words + w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     // --- upload (descs + control in one copy; scenes and init rows per problem)

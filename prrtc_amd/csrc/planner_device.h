@@ -78,6 +78,11 @@ struct PlanControl {
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
     unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit
+    // bit p set while problem p's current round has unclaimed slots: cleared
+    // by the claim of its last slot, set again by the round's commit
+    unsigned long long *open_mask;
+    // bit p set for problems a server serves first (few rounds so far), or null
+    const unsigned long long *young_mask;
     // the running problem furthest along, (round << 32) | p, or 0: its open
     // slots are claimed first and its tasks run at raised wave priority, so
     // the longest problem of a batch is not slowed by the crowd
@@ -91,6 +96,9 @@ constexpr int kSkip = 1, kExtOk = 2, kConnected = 4;
 // favours its oldest problems (they bound the batch's makespan), a server
 // its youngest (they bound request latency).
 constexpr int kPrioNone = 0, kPrioOldest = 1, kPrioYoungest = 2;
+// a server claims the open slots of problems below this many rounds first
+// (its default; the server config sets it)
+constexpr int kYoungRounds = 64;
 __device__ __forceinline__ int task_priority(int policy, int round, bool hot) {
     const int age = round < 8 ? 0 : round < 64 ? 1 : round < 512 ? 2 : 3;
     if (policy == kPrioOldest) return hot ? 3 : (age < 2 ? age : 2);
@@ -191,40 +199,66 @@ __host__ __device__ inline void halton_bases(uint64_t seed, int slot, int dof, f
     for (int i = 0; i < dof; i++) out[i] = primes[i];
 }
 
-// First problem at or after `from` (cyclically) whose bit is set in the
-// running mask, or -1 if none; `count` receives the number of set bits.
-// One load per lane covers 64 mask words.  Wave-uniform results.
-__device__ __forceinline__ int next_running(const unsigned long long *mask, int words, int from, int &count) {
+// One pass over a launch's problem masks, one word per lane (64 words per
+// step): the number of running problems, and the first problem at or after
+// `from` (cyclically) that runs, the first whose current round has
+// unclaimed slots, and the first young one among those (young may be
+// null).  Wave-uniform results.
+struct MaskScan {
+    int running;      // set bits of the running mask
+    int first_run;    // first running problem from `from` on, or -1
+    int open;         // first open problem from `from` on, or -1
+    int open_young;   // first open and young problem from `from` on, or -1
+};
+
+struct FirstFrom {   // first set bit at or after a cursor, wrapping around
+    int a = -1, b = -1;
+    __device__ __forceinline__ void step(unsigned long long m, int w, int w0, unsigned long long hi_part, int base) {
+        const unsigned long long ma = w > w0 ? m : (w == w0 ? m & hi_part : 0ull);
+        const unsigned long long mb = w < w0 ? m : (w == w0 ? m & ~hi_part : 0ull);
+        const uint64_t ba = ballot(ma != 0ull), bb = ballot(mb != 0ull);
+        if (b < 0 && bb) b = first_bit(mb, bb, base);
+        if (a < 0 && ba) a = first_bit(ma, ba, base);
+    }
+    __device__ __forceinline__ int get() const { return a >= 0 ? a : b; }
+    // bit index of the first set bit of the word held by lane ctz(lanes)
+    __device__ __forceinline__ static int first_bit(unsigned long long m, uint64_t lanes, int base) {
+        const int j = __builtin_ctzll(lanes);
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)m, j);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), j);
+        return (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
+    }
+};
+
+__device__ __forceinline__ MaskScan scan_masks(const unsigned long long *run, const unsigned long long *open,
+                                               const unsigned long long *young, int words, int from) {
     const int lane = lane_id();
     const int w0 = from >> 6;
-    const unsigned long long hi_part = ~0ull << (from & 63), lo_part = ~hi_part;
-    int first_a = -1, first_b = -1;   // a: at or after `from`; b: before it (wrap-around)
+    const unsigned long long hi_part = ~0ull << (from & 63);
+    FirstFrom fr, fo, fy;
     int bits = 0;
     for (int base = 0; base < words; base += kWave) {
         const int w = base + lane;
-        const unsigned long long m =
-            w < words ? __hip_atomic_load(mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-        bits += __builtin_popcountll(m);
-        const unsigned long long a = w > w0 ? m : (w == w0 ? m & hi_part : 0ull);
-        const unsigned long long b = w < w0 ? m : (w == w0 ? m & lo_part : 0ull);
-        const uint64_t ba = ballot(a != 0ull), bb = ballot(b != 0ull);
-        if (first_b < 0 && bb) {
-            const int j = __builtin_ctzll(bb);
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, j);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), j);
-            first_b = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
+        unsigned long long r = 0ull, o = 0ull, y = 0ull;
+        if (w < words) {
+            r = __hip_atomic_load(run + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            o = __hip_atomic_load(open + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (young) y = young[w];
         }
-        if (first_a < 0 && ba) {
-            const int j = __builtin_ctzll(ba);
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)a, j);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(a >> 32), j);
-            first_a = (base + j) * 64 + (lo ? __builtin_ctz(lo) : 32 + __builtin_ctz(hi));
-        }
+        bits += __builtin_popcountll(r);
+        fr.step(r, w, w0, hi_part, base);
+        o &= r;
+        fo.step(o, w, w0, hi_part, base);
+        if (young) fy.step(o & y, w, w0, hi_part, base);
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) bits += __shfl_xor(bits, off);
-    count = __builtin_amdgcn_readfirstlane(bits);
-    return first_a >= 0 ? first_a : first_b;
+    MaskScan ms;
+    ms.running = __builtin_amdgcn_readfirstlane(bits);
+    ms.first_run = fr.get();
+    ms.open = fo.get();
+    ms.open_young = fy.get();
+    return ms;
 }
 
 // Nearest node of tree t to q among [0, n): squared L2 as an fmaf chain in

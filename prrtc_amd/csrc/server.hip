@@ -88,6 +88,7 @@ struct Request {
     int entry = -1;
     size_t off = 0;
     int slices = 0;
+    int round = 0;   // planner rounds done, as of the last harvest
     HostProblem hp() const { return HostProblem{robot, start.data(), goals.data(), n_goals, &scene, &st, 0}; }
 };
 
@@ -115,7 +116,9 @@ struct RobotClass {
     PlanState *d_state = nullptr, *h_state = nullptr;
     PlanControl *d_ctrl = nullptr, *h_ctrl_up = nullptr, *h_ctrl_down = nullptr;
     int *d_list = nullptr, *h_list = nullptr;
-    unsigned long long *d_mask = nullptr, *h_mask = nullptr;   // running entries
+    // entry masks, mask_words apart: running | open | young
+    unsigned long long *d_mask = nullptr, *h_mask = nullptr;
+    int mask_words = 0;
     float *d_paths = nullptr;
     int path_rows = 0;
     char *h_stage = nullptr;
@@ -138,7 +141,8 @@ struct RobotClass {
         PRRTC_HIP(hipMalloc(&d_state, sizeof(PlanState) * E));
         PRRTC_HIP(hipMalloc(&d_ctrl, sizeof(PlanControl)));
         PRRTC_HIP(hipMalloc(&d_list, sizeof(int) * E));
-        PRRTC_HIP(hipMalloc(&d_mask, sizeof(unsigned long long) * ((E + 63) / 64)));
+        mask_words = (E + 63) / 64;
+        PRRTC_HIP(hipMalloc(&d_mask, sizeof(unsigned long long) * 3 * mask_words));
         path_rows = E * std::max(1, path_cap - 2);
         PRRTC_HIP(hipMalloc(&d_paths, sizeof(float) * (size_t)path_rows * D));
         // status 0 marks a free entry the planner skips
@@ -148,7 +152,7 @@ struct RobotClass {
         h_ctrl_up = pinned<PlanControl>(1);
         h_ctrl_down = pinned<PlanControl>(1);
         h_list = pinned<int>(E);
-        h_mask = pinned<unsigned long long>((E + 63) / 64);
+        h_mask = pinned<unsigned long long>(3 * mask_words);
         entries.assign(E, nullptr);
         for (int e = 0; e < E; e++) free_entries.insert(e);
         PRRTC_HIP(hipStreamSynchronize(stream));
@@ -278,11 +282,16 @@ void ServerImpl::launch(RobotClass &c) {
     int lds_f4 = 0, ext = 0, max_nnc = 0;
     long long slots = 0;
     const int words = (c.high + 63) / 64;
-    for (int w = 0; w < words; w++) c.h_mask[w] = 0ull;
+    const int W = c.mask_words;
+    for (int w = 0; w < 3 * W; w++) c.h_mask[w] = 0ull;
     for (int e = 0; e < c.high; e++) {
         const Request *rq = c.entries[e];
         if (!rq) continue;
-        c.h_mask[e >> 6] |= 1ull << (e & 63);   // every resident entry is running
+        // every resident entry is running, and open: a launch ends between
+        // tasks, so no round is left with all its slots claimed
+        c.h_mask[e >> 6] |= 1ull << (e & 63);
+        c.h_mask[W + (e >> 6)] |= 1ull << (e & 63);
+        if (rq->round < cfg.young_rounds) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
         lds_f4 = std::max(lds_f4, (rq->lay.scene_floats - 8) / 4);
         ext = std::max(ext, rq->lay.max_ext);
         max_nnc = std::max(max_nnc, rq->st.num_new_configs);
@@ -297,7 +306,9 @@ void ServerImpl::launch(RobotClass &c) {
     hc.mask_words = words;
     hc.prio_policy = kPrioYoungest;
     hc.run_mask = c.d_mask;
-    PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * words, hipMemcpyHostToDevice, s));
+    hc.open_mask = c.d_mask + W;
+    hc.young_mask = cfg.young_rounds > 0 ? c.d_mask + 2 * W : nullptr;
+    PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * 3 * W, hipMemcpyHostToDevice, s));
     PRRTC_HIP(hipMemcpyAsync(c.d_ctrl, c.h_ctrl_up, sizeof(PlanControl), hipMemcpyHostToDevice, s));
     // throughput shape while many problems are resident, latency shape
     // (obstacles split over the task's waves) for the last few
@@ -329,6 +340,7 @@ void ServerImpl::harvest(RobotClass &c, std::vector<std::pair<uint64_t, Finished
         rq->slices++;
         rq->kernel_ns += slice_ns;
         const int status = c.h_state[e].status;
+        rq->round = c.h_state[e].round;
         if (status == kStatusRunning) continue;
         stopped.push_back(e);
         any_solved |= status == kStatusSolved;
@@ -448,6 +460,7 @@ void prrtc_server_config_default(prrtc_server_config *cfg) {
     cfg->max_pending = 65536;
     cfg->slice_us = 1000;
     cfg->path_cap = 1024;
+    cfg->young_rounds = kYoungRounds;
 }
 
 int prrtc_server_create(const prrtc_server_config *cfg, prrtc_server **out) {
@@ -462,6 +475,7 @@ int prrtc_server_create(const prrtc_server_config *cfg, prrtc_server **out) {
             if (cfg->max_pending) c.max_pending = cfg->max_pending;
             if (cfg->slice_us) c.slice_us = cfg->slice_us;
             if (cfg->path_cap) c.path_cap = cfg->path_cap;
+            if (cfg->young_rounds) c.young_rounds = cfg->young_rounds;
         }
         if (c.max_resident < 1 || c.max_resident > (1 << 20) || c.max_pending < 1 || c.slice_us < 1 ||
             c.path_cap < 2 || c.budget_bytes < 4096) {

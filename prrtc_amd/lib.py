@@ -72,7 +72,7 @@ class CTuning(C.Structure):
 class CServerConfig(C.Structure):
     _fields_ = [
         ("budget_bytes", C.c_uint64), ("max_resident", C.c_int32), ("max_pending", C.c_int32),
-        ("slice_us", C.c_int32), ("path_cap", C.c_int32),
+        ("slice_us", C.c_int32), ("path_cap", C.c_int32), ("young_rounds", C.c_int32),
     ]
 
 
@@ -334,8 +334,8 @@ class Server:
     submit() copies the problem and returns a ticket; poll(ticket) returns
     (result, path, timing) once the problem has stopped, else None."""
 
-    def __init__(self, budget_bytes=0, max_resident=0, max_pending=0, slice_us=0, path_cap=0):
-        cfg = CServerConfig(budget_bytes, max_resident, max_pending, slice_us, path_cap)
+    def __init__(self, budget_bytes=0, max_resident=0, max_pending=0, slice_us=0, path_cap=0, young_rounds=0):
+        cfg = CServerConfig(budget_bytes, max_resident, max_pending, slice_us, path_cap, young_rounds)
         h = C.c_void_p()
         _check(load().prrtc_server_create(C.byref(cfg), C.byref(h)), "prrtc_server_create")
         self._h = h

@@ -86,7 +86,7 @@ def test_server_rejects_bad_config_without_gpu():
     from prrtc_amd.lib import CServerConfig, load
     L = load()
     h = C.c_void_p()
-    cfg = CServerConfig(0, -5, 0, 0, 0)      # negative table size
+    cfg = CServerConfig(0, -5, 0, 0, 0, 0)   # negative table size
     assert L.prrtc_server_create(C.byref(cfg), C.byref(h)) == -1
     assert not h.value
     assert L.prrtc_server_create(None, None) == -1

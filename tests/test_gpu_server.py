@@ -44,10 +44,13 @@ def assert_same(r, path, r1, path1, what):
     assert path.tobytes() == path1.tobytes(), what
 
 
-def test_server_same_as_alone(gpu, oracle_robots):
+@pytest.mark.parametrize("young_rounds", [0, -1, 2])
+def test_server_same_as_alone(gpu, oracle_robots, young_rounds):
+    """Claim order (young requests first for 64 or 2 rounds, or plain
+    round-robin) changes no result."""
     reqs = mixed_requests(oracle_robots, 12)
     alone = [gpu.solve(p["robot"], p["start"], p["goals"], p["scene"], st) for p, st in reqs]
-    with gpu.Server(slice_us=150) as srv:
+    with gpu.Server(slice_us=150, young_rounds=young_rounds) as srv:
         tickets = [srv.submit(p, st) for p, st in reqs]
         out = [srv.poll(t) for t in tickets]
         stats = srv.stats()
