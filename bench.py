@@ -16,7 +16,8 @@ each step is one launch of the edge-validation kernel over all edges.
     c0           plain Panda problem (12 spheres, nnc 128, g 32) over seeds:
                  p50/p99 solve ms, success, cost; CPU RRT-Connect beside it
     batched      256 Fetch problems (8-64 mixed obstacles, g 64): plans/s of
-                 one persistent launch vs one launch per problem
+                 one persistent launch vs one launch per problem, and of the
+                 same batch as graph-replayed round grids
     mix          128 Baxter + 256 Panda + 128 Fetch shelf problems (g 128,
                  dynamic domain, balance 1) in one call
     serving      Poisson request stream (Panda:Fetch:Baxter 2:1:1, fresh
@@ -320,6 +321,11 @@ def main():
         t0 = time.perf_counter()
         plain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], pst) for p in feas]
         t_plain = time.perf_counter() - t0
+        # the non-persistent comparator: the same batch as graph-replayed round grids
+        t0 = time.perf_counter()
+        gres, _, gstats = P.solve_batch(feas, bst, graph_rounds=True)
+        t_graph = time.perf_counter() - t0
+        same_graph = all(r["iters"] == g["iters"] and r["cost"] == g["cost"] for r, g in zip(bres, gres))
         sub = min(32, len(feas))
         same = all(r["iters"] == pr[0]["iters"] and pa.tobytes() == pr[1].tobytes() for r, pa, pr in
                    zip(bres[:sub], bpaths[:sub], [P.solve(p["robot"], p["start"], p["goals"], p["scene"], bst)
@@ -332,11 +338,16 @@ def main():
                           "batch_wall_s": t_batch, "plain_wall_s": t_plain,
                           "batch_kernel_ms": bstats["kernel_ns"] / 1e6, "rounds_max": bstats["rounds"],
                           "edge_checks_per_s_batched": bstats["edge_checks"] / (bstats["kernel_ns"] / 1e9),
+                          "plans_per_s_graph_rounds": sum(r["solved"] for r in gres) / t_graph,
+                          "graph_rounds_wall_s": t_graph, "graph_rounds_kernel_ms": gstats["kernel_ns"] / 1e6,
+                          "graph_rounds_identical": bool(same_graph),
                           "nnc_batched": nnc_b, "nnc_plain": nnc_p,
                           "same_settings_identical_results": bool(same),
                           "settings": "fetch, g 64, range 0.5, balance 2, dd off; nnc per mode from the autotuner"}
         log(f"batched {out['batched']['plans_per_s_batched']:.1f} plans/s vs plain "
-            f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x), identical={same}")
+            f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x), identical={same}; "
+            f"graph-replayed rounds {out['batched']['plans_per_s_graph_rounds']:.1f} plans/s, "
+            f"identical={same_graph}")
         for p, r, pa in zip(feas, bres, bpaths):
             if r["solved"]:
                 paths_to_check.append(("fetch", p["scene"], pa, 64))

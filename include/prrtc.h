@@ -156,6 +156,15 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
                       int n_settings, prrtc_result *results, float *paths, int path_cap,
                       prrtc_batch_stats *stats);
 
+/* The same batch without a persistent kernel (no reference counterpart;
+ * the non-persistent comparator of prrtc_solve_batch): every round is a grid
+ * of all running problems' tasks followed by a commit grid, eight rounds per
+ * captured graph, replayed until no problem runs.  Same arguments, same
+ * results; kernel_ns covers the replays. */
+int prrtc_solve_batch_rounds(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
+                             int n_settings, prrtc_result *results, float *paths, int path_cap,
+                             prrtc_batch_stats *stats);
+
 /* Launch shape of the persistent planner per robot (no reference
  * counterpart: the reference launches num_new_configs blocks of 128 threads
  * per problem, src/planning/pRRTC.cu:700-702).  0 = default for each field.

@@ -388,8 +388,13 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
     if (lane == 0) {
         const unsigned long long key = ((unsigned long long)st->round << 32) | (unsigned)p;
         if (status == kStatusRunning) {
-            st->round = st->round + 1;
+            const int round = st->round + 1;
+            st->round = round;
             st->done_slots = 0;
+            // a problem leaves the young class before its next round opens
+            if (ctrl->aged_mask && round == ctrl->young_rounds)
+                __hip_atomic_fetch_or(ctrl->aged_mask + (p >> 6), 1ull << (p & 63), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
             store_release(&st->next_slot, 0);
             __hip_atomic_fetch_or(ctrl->open_mask + (p >> 6), 1ull << (p & 63), __ATOMIC_RELEASE,
                                   __HIP_MEMORY_SCOPE_AGENT);
@@ -472,7 +477,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // running count and the next open problem from the cursor on
                 // (young ones first on a server); closed and finished
                 // problems are skipped through the masks, not visited
-                const MaskScan ms = scan_masks(ctrl->run_mask, ctrl->open_mask, ctrl->young_mask, words, cursor);
+                const MaskScan ms = scan_masks(ctrl, words, cursor);
                 // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
                 // share their counters with the whole grid
@@ -600,6 +605,42 @@ __global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict_
         st->size[1] = d.n_goals;
         st->next_slot = 0;
     }
+}
+
+// The same rounds as plain grids, one kernel pair per round (the
+// non-persistent form, replayed from a captured graph): k_round_tasks runs
+// slot b % max_nnc of problem b / max_nnc on workgroup b, k_round_commit
+// then commits every running problem's round.  Same tasks, same commit
+// order, so the same results as k_plan.
+template <class Robot>
+__global__ __launch_bounds__(kMaxTaskWaves * kWave) __attribute__((amdgpu_waves_per_eu(PRRTC_PLAN_WAVES_PER_EU)))
+void k_round_tasks(const PlanDesc *__restrict__ descs, PlanState *states, int max_nnc, int lds_scene_f4,
+                   int lds_ext) {
+    extern __shared__ float4 lds[];
+    const int p = (int)(blockIdx.x / (unsigned)max_nnc), s = (int)(blockIdx.x % (unsigned)max_nnc);
+    const PlanDesc &d = descs[p];
+    PlanState *st = states + p;
+    if (s >= d.nnc || st->status != kStatusRunning) return;   // the whole workgroup leaves
+    uint64_t *gbuf = reinterpret_cast<uint64_t *>(lds + lds_scene_f4);
+    float *chain = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);
+    float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
+    float *red = uv + 4 * Robot::DOF;
+    int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);
+    TaskCtx ctx;
+    ctx.independent = false;
+    ctx.gbuf = gbuf;
+    bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
+    ctx.obs = ctx.obs.slice(wave_id(), blockDim.x / kWave);
+    plan_task<Robot>(d, st, s, ctx, chain, uv, red, wbuf);
+}
+
+template <class Robot>
+__global__ __launch_bounds__(kWave) void k_round_commit(const PlanDesc *__restrict__ descs, PlanState *states,
+                                                        PlanControl *ctrl, float *paths) {
+    const int p = blockIdx.x;
+    PlanState *st = states + p;
+    if (st->status != kStatusRunning) return;
+    plan_commit<Robot>(descs[p], st, p, ctrl, paths);
 }
 
 // ------------------------------------------------------------------ host
@@ -821,6 +862,59 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
     PRRTC_HIP(hipGetLastError());
 }
 
+// Run rounds until no problem runs: `per_graph` rounds (a task grid and a
+// commit grid each) and a copy of the running mask to the host are
+// captured once as a graph and replayed; the host reads the mask after each
+// replay.  At most max_rounds rounds (every running problem advances one
+// per round and stops at its max_iters).
+void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
+                     int P, int max_nnc, int waves, int lds_scene_f4, int max_ext, const unsigned long long *d_run,
+                     unsigned long long *h_run, int words, long long max_rounds, hipStream_t s) {
+    constexpr int per_graph = 8;
+    const int D = robot_dof(robot);
+    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + kMaxTaskWaves * kWave * sizeof(uint64_t) +
+                       ((size_t)(max_ext + 5) * D + 4 * kMaxTaskWaves + 4) * sizeof(float);
+    const long long tasks = (long long)P * max_nnc;
+    if (tasks > INT32_MAX) throw HipError{hipErrorInvalidValue, "run_round_graph: 2^31 tasks per round"};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PRRTC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < per_graph; r++) {
+        switch (robot) {
+#define PRRTC_ROUND_CASE(ID, R)                                                                                  \
+    case ID:                                                                                                     \
+        hipLaunchKernelGGL(k_round_tasks<R>, dim3((unsigned)tasks), dim3(waves * kWave), lds, s, d_desc, d_state, \
+                           max_nnc, lds_scene_f4, max_ext);                                                      \
+        hipLaunchKernelGGL(k_round_commit<R>, dim3(P), dim3(kWave), 0, s, d_desc, d_state, d_ctrl, d_paths);     \
+        break;
+            PRRTC_ROUND_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
+            PRRTC_ROUND_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
+            PRRTC_ROUND_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
+            PRRTC_ROUND_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+#undef PRRTC_ROUND_CASE
+        }
+    }
+    (void)hipMemcpyAsync(h_run, d_run, sizeof(unsigned long long) * words, hipMemcpyDeviceToHost, s);
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    PRRTC_HIP(ec);
+    struct Release {
+        hipGraph_t &g;
+        hipGraphExec_t &e;
+        ~Release() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } release{graph, exec};
+    PRRTC_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    for (long long done = 0; done < max_rounds; done += per_graph) {
+        PRRTC_HIP(hipGraphLaunch(exec, s));
+        PRRTC_HIP(hipStreamSynchronize(s));
+        bool any = false;
+        for (int w = 0; w < words && !any; w++) any = h_run[w] != 0ull;
+        if (!any) break;
+    }
+}
+
 // Launch shape of each robot's persistent grids (prrtc_set_tuning; 0 = default).
 static std::atomic<int> g_waves_per_cu[4] = {{0}, {0}, {0}, {0}};
 static std::atomic<int> g_task_waves[4] = {{0}, {0}, {0}, {0}};
@@ -866,6 +960,7 @@ int plan_grid(int robot, int waves, long long slots) {
 // A robot group in flight: its problems and where its state lives.
 struct GroupRun {
     std::vector<HostProblem> probs;
+    bool graph_rounds = false;   // graph-replayed round grids instead of one persistent grid
     Workspace *ws = nullptr;
     char *dev = nullptr, *host = nullptr;
     size_t o_state = 0, o_ctrl = 0, o_paths = 0, h_read = 0;
@@ -908,6 +1003,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     std::vector<size_t> h_block(P);
     for (int i = 0; i < P; i++) h_block[i] = hl.take(lay[i].upload_bytes);
     const size_t h_read = hl.take(sizeof(PlanState) * P + sizeof(PlanControl));
+    const size_t h_run = hl.take(sizeof(unsigned long long) * words);
     char *host = ws.need_host(hl.off);
     PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + h_hdr + o_desc);
     for (int i = 0; i < P; i++) {
@@ -942,8 +1038,15 @@ static void start_group(GroupRun &gr, int path_cap) {
     float *d_paths = reinterpret_cast<float *>(dev + o_paths);
     PRRTC_HIP(hipEventRecord(ws.e0, s));
     launch_plan_init(robot, d_desc, d_state, nullptr, P, s);
-    launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, waves, total_slots), waves, lds_f4,
-                      ext_all, s);
+    if (gr.graph_rounds) {
+        long long max_rounds = 0;
+        for (int i = 0; i < P; i++) max_rounds = std::max<long long>(max_rounds, probs[i].st->max_iters + 1);
+        run_round_graph(robot, d_desc, d_state, d_ctrl, d_paths, P, hc->max_nnc, waves, lds_f4, ext_all,
+                        hc->run_mask, reinterpret_cast<unsigned long long *>(host + h_run), words, max_rounds, s);
+    } else {
+        launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, waves, total_slots), waves,
+                          lds_f4, ext_all, s);
+    }
     PRRTC_HIP(hipEventRecord(ws.e1, s));
     // --- queue the readback of states + control
     PRRTC_HIP(hipMemcpyAsync(host + h_read, dev + o_state, sizeof(PlanState) * P, hipMemcpyDeviceToHost, s));
@@ -1028,7 +1131,7 @@ int check_problem(const HostProblem &hp, const char *fn) {
 }
 
 static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
-                      int path_stride, prrtc_batch_stats *stats, const char *fn) {
+                      int path_stride, prrtc_batch_stats *stats, const char *fn, bool graph_rounds = false) {
     const int64_t t0 = now_ns();
     for (auto &hp : probs) {
         int rc = check_problem(hp, fn);
@@ -1048,6 +1151,7 @@ static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, fl
     size_t k = 0;
     for (auto &kv : groups) {
         runs[k].probs = std::move(kv.second);
+        runs[k].graph_rounds = graph_rounds;
         start_group(runs[k++], path_cap);
     }
     for (GroupRun &gr : runs) finish_group(gr, results, paths, path_cap, path_stride, &kernel_ns, &rounds);
@@ -1114,13 +1218,13 @@ int prrtc_get_tuning(int robot, prrtc_tuning *t) {
     return PRRTC_OK;
 }
 
-int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
-                      int n_settings, prrtc_result *results, float *paths, int path_cap,
-                      prrtc_batch_stats *stats) {
-    return guarded("prrtc_solve_batch", [&]() -> int {
+static int solve_batch_call(const char *fn, int n_problems, const prrtc_problem *problems,
+                            const prrtc_settings *settings, int n_settings, prrtc_result *results, float *paths,
+                            int path_cap, prrtc_batch_stats *stats, bool graph_rounds) {
+    return guarded(fn, [&]() -> int {
         if (n_problems < 0 || (n_problems > 0 && (!problems || !settings)) ||
             (n_settings != 1 && n_settings != n_problems)) {
-            set_error("prrtc_solve_batch: invalid argument");
+            set_error("%s: invalid argument", fn);
             return PRRTC_EINVAL;
         }
         if (stats) *stats = prrtc_batch_stats{};
@@ -1131,8 +1235,22 @@ int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc
             probs.push_back(HostProblem{problems[i].robot, problems[i].start, problems[i].goals, problems[i].n_goals,
                                         &problems[i].scene, n_settings == 1 ? settings : settings + i, i});
         }
-        return solve_many(probs, results, paths, path_cap, path_cap * kMaxDof, stats, "prrtc_solve_batch");
+        return solve_many(probs, results, paths, path_cap, path_cap * kMaxDof, stats, fn, graph_rounds);
     });
+}
+
+int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
+                      int n_settings, prrtc_result *results, float *paths, int path_cap,
+                      prrtc_batch_stats *stats) {
+    return solve_batch_call("prrtc_solve_batch", n_problems, problems, settings, n_settings, results, paths,
+                            path_cap, stats, false);
+}
+
+int prrtc_solve_batch_rounds(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
+                             int n_settings, prrtc_result *results, float *paths, int path_cap,
+                             prrtc_batch_stats *stats) {
+    return solve_batch_call("prrtc_solve_batch_rounds", n_problems, problems, settings, n_settings, results,
+                            paths, path_cap, stats, true);
 }
 
 }  // extern "C"

@@ -81,8 +81,12 @@ struct PlanControl {
     // bit p set while problem p's current round has unclaimed slots: cleared
     // by the claim of its last slot, set again by the round's commit
     unsigned long long *open_mask;
-    // bit p set for problems a server serves first (few rounds so far), or null
-    const unsigned long long *young_mask;
+    // young-first claims (a server): bit p set once problem p has done
+    // young_rounds rounds, by the commit that reaches it.  Open problems
+    // without the bit are served first; null: no preference
+    unsigned long long *aged_mask;
+    int young_rounds;
+    int pad2;
     // the running problem furthest along, (round << 32) | p, or 0: its open
     // slots are claimed first and its tasks run at raised wave priority, so
     // the longest problem of a batch is not slowed by the crowd
@@ -202,13 +206,13 @@ __host__ __device__ inline void halton_bases(uint64_t seed, int slot, int dof, f
 // One pass over a launch's problem masks, one word per lane (64 words per
 // step): the number of running problems, and the first problem at or after
 // `from` (cyclically) that runs, the first whose current round has
-// unclaimed slots, and the first young one among those (young may be
-// null).  Wave-uniform results.
+// unclaimed slots, and the first young one among those (none without an
+// aged mask).  Wave-uniform results.
 struct MaskScan {
     int running;      // set bits of the running mask
     int first_run;    // first running problem from `from` on, or -1
     int open;         // first open problem from `from` on, or -1
-    int open_young;   // first open and young problem from `from` on, or -1
+    int open_young;   // first open problem without the aged bit, or -1
 };
 
 struct FirstFrom {   // first set bit at or after a cursor, wrapping around
@@ -230,26 +234,26 @@ struct FirstFrom {   // first set bit at or after a cursor, wrapping around
     }
 };
 
-__device__ __forceinline__ MaskScan scan_masks(const unsigned long long *run, const unsigned long long *open,
-                                               const unsigned long long *young, int words, int from) {
+__device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int words, int from) {
     const int lane = lane_id();
     const int w0 = from >> 6;
     const unsigned long long hi_part = ~0ull << (from & 63);
+    const unsigned long long *aged = ctrl->aged_mask;
     FirstFrom fr, fo, fy;
     int bits = 0;
     for (int base = 0; base < words; base += kWave) {
         const int w = base + lane;
-        unsigned long long r = 0ull, o = 0ull, y = 0ull;
+        unsigned long long r = 0ull, o = 0ull, a = 0ull;
         if (w < words) {
-            r = __hip_atomic_load(run + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            o = __hip_atomic_load(open + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (young) y = young[w];
+            r = __hip_atomic_load(ctrl->run_mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            o = __hip_atomic_load(ctrl->open_mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (aged) a = __hip_atomic_load(aged + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         bits += __builtin_popcountll(r);
         fr.step(r, w, w0, hi_part, base);
         o &= r;
         fo.step(o, w, w0, hi_part, base);
-        if (young) fy.step(o & y, w, w0, hi_part, base);
+        if (aged) fy.step(o & ~a, w, w0, hi_part, base);
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) bits += __shfl_xor(bits, off);

@@ -56,6 +56,9 @@ int plan_task_waves(int robot);
 int plan_batch_waves(int robot);
 int plan_waves_per_cu(int robot);
 int plan_grid(int robot, int waves, long long slots);
+void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
+                     int P, int max_nnc, int waves, int lds_scene_f4, int max_ext, const unsigned long long *d_run,
+                     unsigned long long *h_run, int words, long long max_rounds, hipStream_t s);
 // Resident problems up to which a server slice runs the single-solve shape:
 // as many as a one-wave grid holds with all their tasks in flight.
 int plan_tail_problems(int robot, int max_nnc);

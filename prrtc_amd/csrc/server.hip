@@ -116,7 +116,7 @@ struct RobotClass {
     PlanState *d_state = nullptr, *h_state = nullptr;
     PlanControl *d_ctrl = nullptr, *h_ctrl_up = nullptr, *h_ctrl_down = nullptr;
     int *d_list = nullptr, *h_list = nullptr;
-    // entry masks, mask_words apart: running | open | young
+    // entry masks, mask_words apart: running | open | aged
     unsigned long long *d_mask = nullptr, *h_mask = nullptr;
     int mask_words = 0;
     float *d_paths = nullptr;
@@ -291,7 +291,7 @@ void ServerImpl::launch(RobotClass &c) {
         // tasks, so no round is left with all its slots claimed
         c.h_mask[e >> 6] |= 1ull << (e & 63);
         c.h_mask[W + (e >> 6)] |= 1ull << (e & 63);
-        if (rq->round < cfg.young_rounds) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
+        if (rq->round >= cfg.young_rounds) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
         lds_f4 = std::max(lds_f4, (rq->lay.scene_floats - 8) / 4);
         ext = std::max(ext, rq->lay.max_ext);
         max_nnc = std::max(max_nnc, rq->st.num_new_configs);
@@ -307,7 +307,10 @@ void ServerImpl::launch(RobotClass &c) {
     hc.prio_policy = kPrioYoungest;
     hc.run_mask = c.d_mask;
     hc.open_mask = c.d_mask + W;
-    hc.young_mask = cfg.young_rounds > 0 ? c.d_mask + 2 * W : nullptr;
+    if (cfg.young_rounds > 0) {
+        hc.aged_mask = c.d_mask + 2 * W;
+        hc.young_rounds = cfg.young_rounds;
+    }
     PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * 3 * W, hipMemcpyHostToDevice, s));
     PRRTC_HIP(hipMemcpyAsync(c.d_ctrl, c.h_ctrl_up, sizeof(PlanControl), hipMemcpyHostToDevice, s));
     // throughput shape while many problems are resident, latency shape

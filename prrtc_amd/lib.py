@@ -108,6 +108,8 @@ SIGNATURES = [
                               C.POINTER(CResult), C.c_void_p, C.c_int]),
     ("prrtc_solve_batch", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
                                     C.POINTER(CResult), C.c_void_p, C.c_int, C.POINTER(CBatchStats)]),
+    ("prrtc_solve_batch_rounds", C.c_int, [C.c_int, C.POINTER(CProblem), C.POINTER(CSettings), C.c_int,
+                                           C.POINTER(CResult), C.c_void_p, C.c_int, C.POINTER(CBatchStats)]),
     ("prrtc_set_tuning", C.c_int, [C.c_int, C.POINTER(CTuning)]),
     ("prrtc_get_tuning", C.c_int, [C.c_int, C.POINTER(CTuning)]),
     ("prrtc_server_config_default", None, [C.POINTER(CServerConfig)]),
@@ -272,11 +274,13 @@ def solve(robot, start, goals, scene: Scene, settings: dict | None = None, path_
     return _result_dict(res), path[:res.path_nodes].copy()
 
 
-def solve_batch(problems, settings, path_cap: int = 4096):
+def solve_batch(problems, settings, path_cap: int = 4096, graph_rounds: bool = False):
     """Solve many problems in one persistent launch.
 
     problems: list of dicts {robot, start, goals, scene}; settings: a dict or
-    a list of dicts (one per problem).  Returns (results, paths, stats)."""
+    a list of dicts (one per problem).  Returns (results, paths, stats).
+    graph_rounds: run the rounds as graph-replayed grids instead
+    (prrtc_solve_batch_rounds, the non-persistent comparator)."""
     n = len(problems)
     cps = (CProblem * n)()
     keep = []
@@ -303,8 +307,9 @@ def solve_batch(problems, settings, path_cap: int = 4096):
     res = (CResult * n)()
     paths = np.zeros((n, path_cap * MAX_DOF), dtype=np.float32)
     stats = CBatchStats()
-    rc = load().prrtc_solve_batch(n, cps, sts, n_st, res, paths.ctypes.data, path_cap, C.byref(stats))
-    _check(rc, "prrtc_solve_batch")
+    fn = "prrtc_solve_batch_rounds" if graph_rounds else "prrtc_solve_batch"
+    rc = getattr(load(), fn)(n, cps, sts, n_st, res, paths.ctypes.data, path_cap, C.byref(stats))
+    _check(rc, fn)
     results = [_result_dict(res[i]) for i in range(n)]
     # block i holds path_nodes rows of dof floats, packed
     out_paths = [paths[i, :res[i].path_nodes * dofs[i]].reshape(-1, dofs[i]).copy() for i in range(n)]

@@ -76,6 +76,10 @@ def test_invalid_arguments_fail_without_gpu():
     assert L.prrtc_nearest_async(5, 1, 1024, 10, 1, 1, 1, 1, 1, 1, None) == -1
     assert L.prrtc_nearest_async(7, 1, 1024, 2000, 1, 1, 1, 1, 1, 1, None) == -1
     assert L.prrtc_nearest_async(7, 1, 1024, 10, 1, 1, 1, 9, 1, 1, None) == -1
+    # batch entry points: negative count, settings count neither 1 nor n
+    for fn in (L.prrtc_solve_batch, L.prrtc_solve_batch_rounds):
+        assert fn(-1, None, None, 1, None, None, 16, None) == -1
+        assert fn(2, None, None, 3, None, None, 16, None) == -1
     # tuning ranges
     from prrtc_amd.lib import CTuning
     assert L.prrtc_set_tuning(1, C.byref(CTuning(0, 9, 0))) == -1

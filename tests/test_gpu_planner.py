@@ -125,8 +125,7 @@ def test_paths_revalidate(robot, gpu, oracle_robots):
     print(f"{robot}: {solved}/4 solved, {same} paths identical to the oracle's round-synchronous planner")
 
 
-def test_batch_same_as_alone(gpu, oracle_robots):
-    """Each problem's result is bitwise the same alone and inside a mixed batch."""
+def mixed_batch(oracle_robots):
     from prrtc_amd import scenes as S
     from prrtc_amd.robots import fixed_spheres, limits
     probs, sts = [], []
@@ -143,14 +142,37 @@ def test_batch_same_as_alone(gpu, oracle_robots):
         sts.append(dict(num_new_configs=32 + 16 * (i % 3), granularity=32, range=0.5, balance=i % 3,
                         tree_ratio=1.0 if i % 3 == 2 else 0.5, dynamic_domain=i % 2, max_samples=40000,
                         max_iters=150, seed=i))
+    return probs, sts
+
+
+def assert_same_result(r, path, r1, path1, what):
+    for k in ("solved", "iters", "start_tree_size", "goal_tree_size", "path_length", "edge_checks"):
+        assert r[k] == r1[k], (what, k, r[k], r1[k])
+    assert np.float32(r["cost"]) == np.float32(r1["cost"]), what
+    assert path.tobytes() == path1.tobytes(), what
+
+
+def test_batch_same_as_alone(gpu, oracle_robots):
+    """Each problem's result is bitwise the same alone and inside a mixed batch."""
+    probs, sts = mixed_batch(oracle_robots)
     results, paths, stats = gpu.solve_batch(probs, sts)
     assert stats["solved"] == sum(r["solved"] for r in results)
     for p, st, r, path in zip(probs, sts, results, paths):
         r1, path1 = gpu.solve(p["robot"], p["start"], p["goals"], p["scene"], st)
-        for k in ("solved", "iters", "start_tree_size", "goal_tree_size", "path_length", "edge_checks"):
-            assert r[k] == r1[k], (p["robot"], k)
-        assert np.float32(r["cost"]) == np.float32(r1["cost"])
-        assert path.tobytes() == path1.tobytes()
+        assert_same_result(r, path, r1, path1, p["robot"])
+
+
+def test_graph_rounds_same_as_persistent(gpu, oracle_robots):
+    """prrtc_solve_batch_rounds (one task grid and one commit grid per round,
+    replayed from a captured graph) returns bitwise the persistent batch's
+    results, with several graph replays (problems run up to 150 rounds)."""
+    probs, sts = mixed_batch(oracle_robots)
+    results, paths, stats = gpu.solve_batch(probs, sts)
+    gres, gpaths, gstats = gpu.solve_batch(probs, sts, graph_rounds=True)
+    assert gstats["solved"] == stats["solved"] and gstats["rounds"] == stats["rounds"]
+    assert stats["rounds"] > 8, "every problem finished within one graph replay"
+    for p, r, path, r1, path1 in zip(probs, gres, gpaths, results, paths):
+        assert_same_result(r, path, r1, path1, p["robot"])
 
 
 def test_forced_failures(gpu, oracle_robots):
@@ -177,14 +199,11 @@ def test_forced_failures(gpu, oracle_robots):
 def test_task_waves_same_results(waves, gpu, oracle_robots):
     """A task spread over several waves (shared nearest-node scans, obstacle
     tables split between the waves of each validation pass) returns bitwise
-    the single-wave result, alone and in a batch that switches from the
-    one-wave throughput grid to the multi-wave latency grid for its last
-    problems."""
+    the single-wave result, alone and inside a one-wave batch."""
     from prrtc_amd import scenes as S
     from prrtc_amd.robots import fixed_spheres, limits
     cases = []
-    # 12 point-robot problems make their group large enough (12 x 48 slots
-    # over a 512-task latency grid) for the batch to run two launch shapes
+    # 9 more point-robot problems make the point-robot group a real batch
     for i in range(21):
         robot = ["sphere", "panda", "fetch", "baxter"][i % 4] if i < 12 else "sphere"
         R = oracle_robots[robot]
