@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--edges", type=int, default=1_000_000)
     ap.add_argument("--granularity", type=int, default=128)
     ap.add_argument("--plans", type=int, default=16, help="single-problem plans for p50/p99")
-    ap.add_argument("--c0-seeds", type=int, default=32)
+    ap.add_argument("--c0-seeds", type=int, default=128)
     ap.add_argument("--batch", type=int, default=256, help="Fetch problems of the batched vs plain comparison")
     ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
@@ -381,38 +381,16 @@ def main():
 
     if not args.skip_serving:
         # ---------------------------------------------------------- serving
+        from prrtc_amd import loadgen
         n_req = max(1, int(args.serve_rate * args.serve_seconds))
         log(f"serving: {n_req} requests at {args.serve_rate:.0f}/s")
-        reqs = [workloads.serving_request(i, seed=0) for i in range(n_req)]
-        sst = {r: dict(num_new_configs=64, granularity=64, range=0.5, balance=2, tree_ratio=1.0,
-                       dynamic_domain=0, max_samples=50000, max_iters=3000, seed=1)
-               for r in ("panda", "fetch", "baxter")}
-        arrivals = np.cumsum(np.random.default_rng(11).exponential(1.0 / args.serve_rate, n_req))
-        with P.Server(budget_bytes=16 << 30, slice_us=1000, young_rounds=args.serve_young) as srv:
-            t_start = time.perf_counter()
-            tickets = []
-            for a, rq in zip(arrivals, reqs):
-                delay = t_start + a - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                tickets.append(srv.submit(rq, sst[rq["robot"]]))
-            outs = [srv.poll(t) for t in tickets]
-            t_end = time.perf_counter()
-            sstats = srv.stats()
-        lat = [(tm["queued_ns"] + tm["resident_ns"]) / 1e6 for _, _, tm in outs]
-        queued = [tm["queued_ns"] / 1e6 for _, _, tm in outs]
-        lat_solved = [l for l, (r, _, _) in zip(lat, outs) if r["solved"]]
-        n_s = sum(r["solved"] for r, _, _ in outs)
-        out["serving"] = {"requests": n_req, "rate_per_s": args.serve_rate, "solved": n_s,
-                          "plans_per_s": n_s / (t_end - t_start), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                          "p50_ms_solved": pct(lat_solved, 50), "p99_ms_solved": pct(lat_solved, 99),
-                          "admission_wait_p50_ms": pct(queued, 50), "admission_wait_p99_ms": pct(queued, 99),
-                          "slices": sstats["slices"], "peak_resident_mib": sstats["peak_resident_bytes"] / 2**20,
-                          "settings": "g 64, nnc 64, range 0.5, balance 2, max_iters 3000; 1 ms slices, 16 GiB budget, "
-                                      f"young_rounds {args.serve_young or 64}"}
-        log(f"serving: {n_s}/{n_req} solved, p50 {out['serving']['p50_ms']:.1f} ms, "
-            f"p99 {out['serving']['p99_ms']:.1f} ms, {out['serving']['plans_per_s']:.1f} plans/s")
-        for rq, (r, pa, _) in zip(reqs[:200], outs[:200]):
+        reqs = loadgen.request_pool(n_req)
+        sv = loadgen.run_stream(reqs, args.serve_rate, args.serve_seconds, young_rounds=args.serve_young)
+        outs = sv.pop("_outs")
+        out["serving"] = sv
+        log(f"serving: {sv['solved']}/{n_req} solved, p50 {sv['p50_ms']:.1f} ms, "
+            f"p99 {sv['p99_ms']:.1f} ms, {sv['plans_per_s']:.1f} plans/s")
+        for rq, r, pa, _ in outs[:200]:
             if r["solved"]:
                 paths_to_check.append((rq["robot"], rq["scene"], pa, 64))
 

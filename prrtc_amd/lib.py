@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libprrtc.so")
 
 PRRTC_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUSY", -6: "EAGAIN"}
+PRRTC_EBUSY = -5
 PRRTC_EAGAIN = -6
 MAX_DOF = 14
 
@@ -126,7 +127,11 @@ _lib = None
 
 
 class PrrtcError(RuntimeError):
-    pass
+    """A call returned a PRRTC_E* status; `code` holds it."""
+
+    def __init__(self, msg: str, code: int | None = None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load():
@@ -147,7 +152,7 @@ def load():
 def _check(rc: int, what: str):
     if rc != PRRTC_OK:
         msg = load().prrtc_last_error().decode()
-        raise PrrtcError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+        raise PrrtcError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}", rc)
 
 
 def robot_id(robot) -> int:
