@@ -416,6 +416,13 @@ def main():
                                          f"{threads} threads, static partition",
                                "seconds": dt, "verdict_agreement": agree}
         log(f"cpu: {len(sample) / dt:.0f} edge checks/s on {threads} threads, agreement {agree:.4f}")
+        one = np.ascontiguousarray(edges[: max(1, args.cpu_edges // 25)])
+        t0 = time.perf_counter()
+        model("panda").validate_edges(scene, one, g, threads=1)
+        dt1 = time.perf_counter() - t0
+        out["cpu_baseline_1thread"] = {"value": len(one) / dt1, "unit": "edge checks/s", "cores": 1,
+                                       "sample": f"first {len(one)} of the same edges", "seconds": dt1}
+        log(f"cpu: {len(one) / dt1:.0f} edge checks/s on 1 thread")
         if "c0" in out:
             c0 = workloads.c0_problem(0)
             times, solved = [], 0
@@ -449,6 +456,32 @@ def main():
                                          "kind": "project CPU RRT-Connect (vamp unavailable offline)",
                                          "settings": "fetch, g 64, range 0.5, max 5000 samples per problem"}
             log(f"cpu rrt-connect c2: {cpu_solved}/{len(fprobs)} solved in {dt:.1f} s on {threads} threads")
+        if "mix" in out:
+            # the uneven mix through the CPU RRT-Connect (its own robot per
+            # problem), bounded at 5000 samples per problem
+            from concurrent.futures import ThreadPoolExecutor
+
+            def cpu_mix(p):
+                return model(p["robot"]).rrt_connect(p["scene"], p["start"], p["goals"][0], 128, 0.5, 5000,
+                                                     1)[1].solved
+
+            for r in sorted({p["robot"] for p in mprobs}):
+                model(r)   # built before the worker threads share the table
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                mix_solved = list(ex.map(cpu_mix, mprobs))
+            dt = time.perf_counter() - t0
+            per = {}
+            for p, sv in zip(mprobs, mix_solved):
+                d = per.setdefault(p["robot"], {"problems": 0, "solved": 0})
+                d["problems"] += 1
+                d["solved"] += int(sv)
+            out["cpu_rrt_connect_c3"] = {"problems": len(mprobs), "solved": int(sum(mix_solved)),
+                                         "plans_per_s": sum(mix_solved) / dt, "wall_s": dt, "cores": threads,
+                                         "per_robot": per,
+                                         "kind": "project CPU RRT-Connect (vamp unavailable offline)",
+                                         "settings": "g 128, range 0.5, max 5000 samples per problem"}
+            log(f"cpu rrt-connect c3: {sum(mix_solved)}/{len(mprobs)} solved in {dt:.1f} s on {threads} threads")
         bad = 0
         for robot, sc, path, gg in paths_to_check:
             bad += model(robot).revalidate(sc, path, gg) != -1
