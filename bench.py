@@ -115,7 +115,8 @@ def main():
     g = args.granularity
     tuned = load_tuned()
     for robot, t in tuned.get("launch", {}).items():
-        P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0))
+        P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0),
+                     t.get("crowd_above", 0))
 
     # ---------------------------------------------------------------- inputs
     log("building the scene and the edges")

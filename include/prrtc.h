@@ -179,12 +179,21 @@ int prrtc_solve_batch_rounds(int n_problems, const prrtc_problem *problems, cons
  *   batch_waves_per_task  the same for prrtc_solve_batch and for server
  *                         slices with many resident problems (1..4;
  *                         PRRTC_BATCH_WAVES, else 4)
+ *   crowd_above           with more running problems than this, a batch or
+ *                         server workgroup claims one slot per wave and each
+ *                         wave runs its own task with the whole scene
+ *                         (throughput); with fewer its waves share one slot
+ *                         (latency).  < 0: never; 0: PRRTC_CROWD_ABOVE, else
+ *                         CUs x 4 / num_new_configs (the planner holds one
+ *                         wave per SIMD).  prrtc_get_tuning returns the
+ *                         setting (0 = that default)
  * robot = -1 sets every robot.  Applies to later prrtc_solve,
  * prrtc_solve_batch and server slices; results do not depend on it. */
 typedef struct prrtc_tuning {
     int32_t waves_per_cu;
     int32_t waves_per_task;
     int32_t batch_waves_per_task;
+    int32_t crowd_above;
 } prrtc_tuning;
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t);

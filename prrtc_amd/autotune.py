@@ -20,7 +20,7 @@ change results; num_new_configs does (it is a planner setting), so every
 candidate is timed on the same problems and the choice is by solved plans
 per second.  The result is JSON, read by bench.py:
 
-    {"launch": {robot: {"waves_per_cu", "waves_per_task", "batch_waves_per_task"}},
+    {"launch": {robot: {"waves_per_cu", "waves_per_task", "batch_waves_per_task", "crowd_above"}},
      "batch_nnc": {robot: n}, "plain_nnc": {robot: n},
      "measured": {...every candidate's plans/s...}}
 
@@ -80,16 +80,20 @@ def tune_robot(P, robot, n, pilot_iters, max_iters, plain_n):
         for wpc in (4, 8):
             if wpc < waves:
                 continue
-            P.set_tuning(robot, waves_per_cu=wpc, waves_per_task=waves, batch_waves_per_task=waves)
-            for nnc in (32, 64, 128, 256):
-                st = dict(base, num_new_configs=nnc, max_iters=max_iters)
-                dt, (res, _, _) = timed(lambda: P.solve_batch(feas, st))
-                rate = sum(r["solved"] for r in res) / dt
-                measured["batch"].append({"waves_per_task": waves, "waves_per_cu": wpc, "nnc": nnc,
-                                          "plans_per_s": rate, "wall_s": dt})
-                log(f"{robot} batch waves {waves} waves/CU {wpc} nnc {nnc}: {rate:.1f} plans/s")
-                if best_b is None or rate > best_b[0]:
-                    best_b = (rate, waves, wpc, nnc)
+            # crowd claims (one slot per wave while many problems run) need
+            # several waves per workgroup; -1 turns them off, 0 is the default
+            for crowd in ((-1, 0) if waves > 1 else (-1,)):
+                P.set_tuning(robot, waves_per_cu=wpc, waves_per_task=waves, batch_waves_per_task=waves,
+                             crowd_above=crowd)
+                for nnc in (32, 64, 128, 256):
+                    st = dict(base, num_new_configs=nnc, max_iters=max_iters)
+                    dt, (res, _, _) = timed(lambda: P.solve_batch(feas, st))
+                    rate = sum(r["solved"] for r in res) / dt
+                    measured["batch"].append({"waves_per_task": waves, "waves_per_cu": wpc, "crowd_above": crowd,
+                                              "nnc": nnc, "plans_per_s": rate, "wall_s": dt})
+                    log(f"{robot} batch waves {waves} waves/CU {wpc} crowd {crowd} nnc {nnc}: {rate:.1f} plans/s")
+                    if best_b is None or rate > best_b[0]:
+                        best_b = (rate, waves, wpc, nnc, crowd)
     sub = feas[:plain_n]
     best_p = None
     for waves in (2, 4):
@@ -103,7 +107,8 @@ def tune_robot(P, robot, n, pilot_iters, max_iters, plain_n):
             if best_p is None or rate > best_p[0]:
                 best_p = (rate, waves, nnc)
     P.set_tuning(robot, 0, 0, 0)
-    launch = {"waves_per_cu": best_b[2], "waves_per_task": best_p[1], "batch_waves_per_task": best_b[1]}
+    launch = {"waves_per_cu": best_b[2], "waves_per_task": best_p[1], "batch_waves_per_task": best_b[1],
+              "crowd_above": best_b[4]}
     return launch, best_b[3], best_p[2], measured
 
 

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kMaxTaskWaves * kWave) void k_nearest(const float *
 #pragma unroll
         for (int d = 0; d < D; d++) q[d] = queries[(size_t)k * D + d];
         float best;
-        const int i = nearest<D>(tree, cap, n, q, best, red);
+        const int i = nearest<D>(tree, cap, n, q, best, red, team_workgroup());
         if (threadIdx.x == 0) {
             idx[k] = i;
             dist2[k] = best;

@@ -26,12 +26,12 @@ namespace prrtc {
 // Combine one value per wave across the workgroup: the OR (want_min false)
 // or the minimum of `v`.  `buf` holds 2 * kMaxTaskWaves ints in LDS, used by
 // alternate calls (`parity`), so one barrier per call suffices.
-__device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int parity) {
-    const int W = blockDim.x / kWave;
+__device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int parity, const Team &tm) {
+    const int W = tm.waves;
     if (W == 1) return v;
     int *b = buf + parity * kMaxTaskWaves;
-    if (lane_id() == 0) b[wave_id()] = v;
-    __syncthreads();
+    if (lane_id() == 0) b[tm.wave] = v;
+    tm.sync();
     int r = b[0];
     for (int w = 1; w < W; w++) r = want_min ? min(r, b[w]) : (r | b[w]);
     return r;
@@ -46,8 +46,9 @@ __device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int 
 // are those of a single wave that sees every obstacle.
 struct TaskCtx : CheckCtx {
     uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
+    int waves;        // waves of the task: the whole workgroup, or 1
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
-        const int W = blockDim.x / kWave;
+        const int W = waves;
         if (W == 1) return;
         gbuf[threadIdx.x] = gate;
         __syncthreads();
@@ -60,7 +61,8 @@ struct TaskCtx : CheckCtx {
 };
 
 // One task: slot s of problem p in the current round (pRRTC.cu:342-551),
-// run by every thread of the workgroup (1..kMaxTaskWaves waves).
+// run by the threads of `tm`: the whole workgroup (1..kMaxTaskWaves waves)
+// or one wave on its own.
 //
 // The steered edge and the connect chain are validated by one loop with a
 // single call site of the robot's check sequence (the sequence is large; one
@@ -71,10 +73,10 @@ struct TaskCtx : CheckCtx {
 // the connect step; red and wbuf are the cross-wave reduction buffers.
 template <class Robot>
 __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx, float *chain, float *uv,
-                          float *red, int *wbuf) {
+                          float *red, int *wbuf, const Team &tm) {
 #pragma clang fp contract(off)
     constexpr int D = Robot::DOF;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = tm.tid, NT = tm.nt;
     const int lane = lane_id();
     const int iter = st->round;
     const int n0 = st->size[0], n1 = st->size[1];
@@ -100,7 +102,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
     }
     // every wave has read the slot's Halton state and tree before thread 0
     // advances them
-    if (NT > kWave) __syncthreads();
+    if (tm.waves > 1) tm.sync();
     if (tid == 0) {
 #pragma unroll
         for (int k = 0; k < D; k++) { hs[D + k] = h_n[k]; hs[2 * D + k] = h_d[k]; }
@@ -110,7 +112,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
     // nearest node in the target tree (pRRTC.cu:384-418)
     const float *tn = d.nodes + (size_t)t * D * d.cap;
     float bsq;
-    const int nn = nearest<D>(tn, d.cap, t == 0 ? n0 : n1, sample, bsq, red);
+    const int nn = nearest<D>(tn, d.cap, t == 0 ? n0 : n1, sample, bsq, red, tm);
     const float dist = sqrtf(bsq);
     const float scale = fminf(1.0f, d.range / dist);
     int flags = 0, n_ok = 0, nn_o = 0;
@@ -127,7 +129,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
             chain[tid] = c;
             delta[tid] = (c - a) / gf;
         }
-        __syncthreads();
+        tm.sync();
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
         const int passes = (g + kWave - 1) / kWave;
@@ -161,7 +163,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
             if (phase == 1) {
                 // first colliding sub-edge: the smallest any wave found
                 const int wfirst = cm ? __builtin_amdgcn_readlane(key, __builtin_ctzll(cm)) : 0x7fffffff;
-                const int first = across_waves(wfirst, true, wbuf, npass++ & 1);
+                const int first = across_waves(wfirst, true, wbuf, npass++ & 1, tm);
                 if (first != 0x7fffffff) {
                     n_ok = first;
                     break;
@@ -170,7 +172,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
                 if (base >= total) { n_ok = n_chain; break; }
                 continue;
             }
-            if (across_waves(cm ? 1 : 0, false, wbuf, npass++ & 1)) break;   // steered edge collides
+            if (across_waves(cm ? 1 : 0, false, wbuf, npass++ & 1, tm)) break;   // steered edge collides
             if (++p < passes) continue;
             // steered edge free: connect to the other tree (pRRTC.cu:478-518)
             flags |= kExtOk;
@@ -178,7 +180,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
 #pragma unroll
             for (int c = 0; c < D; c++) cfg[c] = chain[c];
             const float *on = d.nodes + (size_t)o * D * d.cap;
-            nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq, red);
+            nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq, red, tm);
             const float dist_o = sqrtf(bsq);
             n_ext = (unsigned)ceilf(dist_o / d.range);
             n_chain = (int)min(n_ext, (unsigned)d.max_ext);
@@ -187,7 +189,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
                 vecv[tid] = vec;
                 step[tid] = vec / gf;
             }
-            __syncthreads();
+            tm.sync();
             // chain configurations by sequential adds, thread i computes c_i
             for (int i0 = 1; i0 <= n_chain; i0 += NT) {
                 const int i = i0 + tid;
@@ -201,7 +203,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
                     }
                 }
             }
-            __syncthreads();
+            tm.sync();
             total = n_chain * g;
             if (total == 0) { n_ok = 0; break; }
             phase = 1;
@@ -219,7 +221,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
                 }
             }
         }
-        __syncthreads();
+        tm.sync();
     }
     if (tid == 0) {
         meta[0] = t;
@@ -419,23 +421,27 @@ __global__ __launch_bounds__(kMaxTaskWaves * kWave) __attribute__((amdgpu_waves_
 void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *ctrl, float *paths, int lds_scene_f4,
             int lds_ext) {
     extern __shared__ float4 lds[];
-    // LDS: scene tables | gate exchange | chain rows | uniform vectors |
-    // cross-wave buffers | the leader's decision
+    // LDS: scene tables | gate exchange | one task area per wave (chain
+    // rows, uniform vectors, cross-wave buffers) | the leader's decision.  A
+    // task of the whole workgroup uses area 0; after a crowd claim wave w
+    // runs its own task in area w.
+    constexpr int D = Robot::DOF;
+    const int W = blockDim.x / kWave;
     uint64_t *gbuf = reinterpret_cast<uint64_t *>(lds + lds_scene_f4);       // kMaxTaskWaves * kWave
-    float *chain = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);
-    float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
-    float *red = uv + 4 * Robot::DOF;                                 // 2 * kMaxTaskWaves
-    int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);     // 2 * kMaxTaskWaves
-    int *ctl = wbuf + 2 * kMaxTaskWaves;                              // the leader's decision: p, s
+    float *areas = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);
+    const int area = task_area_floats(D, lds_ext);
+    int *ctl = reinterpret_cast<int *>(areas + (size_t)W * area);   // p, first slot, slots, priority, crowd
     TaskCtx ctx;
     ctx.independent = false;
     ctx.gbuf = gbuf;
-    const bool leader = wave_id() == 0;   // wave 0 claims the tasks and commits the rounds
+    Obstacles scene_obs;   // the staged scene, all tables
+    const bool leader = wave_id() == 0;   // wave 0 claims the tasks
     int staged = -1;
     int idle = 0, backoff = 1;
     const int P = ctrl->n_problems;
     const int prio_policy = ctrl->prio_policy;
     const int max_nnc = ctrl->max_nnc;
+    const int crowd_above = W > 1 ? ctrl->crowd_above : 0;
     const unsigned long long slice = ctrl->slice_ticks;
     unsigned long long t0 = 0;
     if (slice && leader) {
@@ -454,24 +460,28 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
     int cursor = (int)(blockIdx.x % (unsigned)P);   // each workgroup starts at its own problem
     while (true) {
         if (leader) {
-            int p_out = -1, s_out = -1, hot = 0;   // p_out < 0: the workgroup leaves
-            // Claim slot s of problem q if q runs and has unclaimed slots in
-            // this round.  Plain loads come first, so the counters' cache
-            // lines stay cheap to share; a claim counts only below nnc and
-            // finished problems are never incremented, so it cannot wrap.
-            // The claim of the last slot closes the problem in the open mask
-            // (before that task can finish, so before the commit reopens it).
-            auto try_claim = [&](int q) -> int {
+            int p_out = -1, s_out = -1, n_out = 1, hot = 0, crowd = 0;   // p_out < 0: the workgroup leaves
+            // Claim up to k consecutive slots of problem q if q runs and has
+            // unclaimed slots in this round; returns the first slot (-1: none)
+            // and their number in `got`.  Plain loads come first, so the
+            // counters' cache lines stay cheap to share; a claim counts only
+            // below nnc and finished problems are never incremented, so it
+            // cannot wrap.  The claim of the last slot closes the problem in
+            // the open mask (before that task can finish, so before the
+            // commit reopens it).
+            auto try_claim = [&](int q, int k, int &got) -> int {
                 PlanState *st = states + q;
                 const int nnc = descs[q].nnc;
                 const bool live = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
                 const bool open = live && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
                                               &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
-                const int s = open ? wave_atomic_add(&st->next_slot, 1) : nnc;
-                if (s == nnc - 1 && lane_id() == 0)
+                const int s = open ? wave_atomic_add(&st->next_slot, k) : nnc;
+                if (s < 0 || s >= nnc) return -1;
+                got = min(k, nnc - s);
+                if (s + got == nnc && lane_id() == 0)
                     __hip_atomic_fetch_and(ctrl->open_mask + (q >> 6), ~(1ull << (q & 63)), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
-                return s >= 0 && s < nnc ? s : -1;
+                return s;
             };
             while (true) {
                 // running count and the next open problem from the cursor on
@@ -485,14 +495,20 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // a time-sliced launch ends between tasks; unfinished rounds
                 // resume in the next launch from the state kept in device memory
                 if (slice && __builtin_amdgcn_s_memrealtime() - t0 > slice) break;
-                // a batch serves the problem furthest along first
+                // many running problems (a crowd): each wave of the workgroup
+                // takes its own slot and runs it alone, which spends no work
+                // on splitting; few: the waves share one slot, which shortens
+                // the round
+                const int k = crowd_above > 0 && ms.running > crowd_above ? W : 1;
+                int got = 1;
+                // a batch serves the problem furthest along first, on all waves
                 const unsigned long long hk =
                     prio_policy == kPrioOldest
                         ? __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                         : 0ull;
                 const int ph = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
                 if (ph >= 0 && ph < P) {
-                    const int s = try_claim(ph);
+                    const int s = try_claim(ph, 1, got);
                     if (s >= 0) {
                         p_out = ph;
                         s_out = s;
@@ -504,13 +520,15 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 }
                 const int q = __builtin_amdgcn_readfirstlane(ms.open_young >= 0 ? ms.open_young : ms.open);
                 if (q >= 0) {
-                    const int s = try_claim(q);
+                    const int s = try_claim(q, k, got);
                     if (s >= 0) {
                         // the cursor stays: the next task is likely from the
                         // same problem, whose scene is already staged
                         cursor = q;
                         p_out = q;
                         s_out = s;
+                        n_out = got;
+                        crowd = k > 1;
                         idle = 0;
                         backoff = 1;
                         break;
@@ -525,18 +543,20 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // until a commit reopens one (exponential back-off, up to
                 // 4 x 64 x 64 cycles)
                 const int r = __builtin_amdgcn_readfirstlane(ms.first_run);
-                const int s = r >= 0 ? try_claim(r) : -1;
+                const int s = r >= 0 ? try_claim(r, k, got) : -1;
                 if (s >= 0) {
                     cursor = r;
                     p_out = r;
                     s_out = s;
+                    n_out = got;
+                    crowd = k > 1;
                     idle = 0;
                     backoff = 1;
                     break;
                 }
                 cursor = r + 1 < P ? r + 1 : 0;
                 if (++idle >= ms.running) {
-                    for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(64);
+                    for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(64);
                     backoff = backoff < 4 ? 2 * backoff : 4;
                     idle = 0;
                 }
@@ -545,14 +565,17 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             if (lane_id() == 0) {
                 ctl[0] = p_out;
                 ctl[1] = s_out;
-                ctl[2] = prio;
+                ctl[2] = n_out;
+                ctl[3] = prio;
+                ctl[4] = crowd;
             }
         }
         __syncthreads();
-        const int p = ctl[0], s = ctl[1];
+        const int p = ctl[0], s = ctl[1], n_slots = ctl[2];
+        const bool crowd = ctl[4] != 0;
         if (p < 0) break;
         // issue priority of this task's waves on their SIMDs
-        switch (ctl[2]) {
+        switch (ctl[3]) {
             case 3: __builtin_amdgcn_s_setprio(3); break;
             case 2: __builtin_amdgcn_s_setprio(2); break;
             case 1: __builtin_amdgcn_s_setprio(1); break;
@@ -561,17 +584,32 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
         const PlanDesc &d = descs[p];
         PlanState *st = states + p;
         if (staged != p) {   // every wave is past its last use of the staged scene
-            bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
-            ctx.obs = ctx.obs.slice(wave_id(), blockDim.x / kWave);   // this wave's share of the obstacles
+            bind_scene<true>(d.scene, d.scene_f4, lds, scene_obs);
             staged = p;
         }
-        plan_task<Robot>(d, st, s, ctx, chain, uv, red, wbuf);
-        __threadfence();
-        __syncthreads();   // the whole task is written, and ctl read, before the leader goes on
-        if (leader) {
+        // Without a crowd claim every wave runs the one slot and checks its
+        // share of the obstacles; after one, wave w runs slot s + w on its
+        // own with the whole scene (waves beyond the claimed slots idle).
+        // One call site of plan_task and of plan_commit either way.
+        const int w = wave_id();
+        const bool runs = !crowd || w < n_slots;
+        if (runs) {
+            ctx.obs = crowd ? scene_obs : scene_obs.slice(w, W);
+            ctx.waves = crowd ? 1 : W;
+            float *a = areas + (crowd ? (size_t)w * area : 0);
+            plan_task<Robot>(d, st, crowd ? s + w : s, ctx, a, a + (lds_ext + 1) * D, a + (lds_ext + 5) * D,
+                             reinterpret_cast<int *>(a + (lds_ext + 5) * D + 2 * kMaxTaskWaves),
+                             crowd ? team_wave() : team_workgroup());
+            __threadfence();
+        }
+        // a shared task is written whole before its slot is counted
+        if (!crowd) __syncthreads();
+        // the wave that counts the round's last slot commits the round
+        if (crowd ? runs : leader) {
             const int prev = wave_atomic_add(&st->done_slots, 1);
             if (prev == d.nnc - 1) plan_commit<Robot>(d, st, p, ctrl, paths);
         }
+        if (crowd) __syncthreads();   // ctl read by every wave before the leader claims again
     }
 }
 
@@ -621,17 +659,17 @@ void k_round_tasks(const PlanDesc *__restrict__ descs, PlanState *states, int ma
     const PlanDesc &d = descs[p];
     PlanState *st = states + p;
     if (s >= d.nnc || st->status != kStatusRunning) return;   // the whole workgroup leaves
+    constexpr int D = Robot::DOF;
     uint64_t *gbuf = reinterpret_cast<uint64_t *>(lds + lds_scene_f4);
-    float *chain = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);
-    float *uv = chain + (size_t)(lds_ext + 1) * Robot::DOF;
-    float *red = uv + 4 * Robot::DOF;
-    int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);
+    float *a = reinterpret_cast<float *>(gbuf + kMaxTaskWaves * kWave);   // task area 0 (k_plan's layout)
     TaskCtx ctx;
     ctx.independent = false;
     ctx.gbuf = gbuf;
+    ctx.waves = blockDim.x / kWave;
     bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
-    ctx.obs = ctx.obs.slice(wave_id(), blockDim.x / kWave);
-    plan_task<Robot>(d, st, s, ctx, chain, uv, red, wbuf);
+    ctx.obs = ctx.obs.slice(wave_id(), ctx.waves);
+    plan_task<Robot>(d, st, s, ctx, a, a + (lds_ext + 1) * D, a + (lds_ext + 5) * D,
+                     reinterpret_cast<int *>(a + (lds_ext + 5) * D + 2 * kMaxTaskWaves), team_workgroup());
 }
 
 template <class Robot>
@@ -840,13 +878,16 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
     PRRTC_HIP(hipGetLastError());
 }
 
+// Dynamic LDS of a planner grid: scene, gate exchange, one task area per
+// wave, the leader's decision (the layout k_plan and k_round_tasks use).
+static size_t plan_lds_bytes(int D, int lds_scene_f4, int max_ext, int waves) {
+    return (size_t)lds_scene_f4 * sizeof(float4) + kMaxTaskWaves * kWave * sizeof(uint64_t) +
+           ((size_t)waves * task_area_floats(D, max_ext) + kCtlInts) * sizeof(float);
+}
+
 void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
                        int grid, int waves, int lds_scene_f4, int max_ext, hipStream_t s) {
-    const int D = robot_dof(robot);
-    // scene, gate exchange, chain rows, 4 uniform vectors, cross-wave
-    // buffers, leader decision
-    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + kMaxTaskWaves * kWave * sizeof(uint64_t) +
-                       ((size_t)(max_ext + 5) * D + 4 * kMaxTaskWaves + 4) * sizeof(float);
+    const size_t lds = plan_lds_bytes(robot_dof(robot), lds_scene_f4, max_ext, waves);
     switch (robot) {
 #define PRRTC_SLICE_CASE(ID, R)                                                                              \
     case ID:                                                                                                 \
@@ -871,9 +912,7 @@ void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, Plan
                      int P, int max_nnc, int waves, int lds_scene_f4, int max_ext, const unsigned long long *d_run,
                      unsigned long long *h_run, int words, long long max_rounds, hipStream_t s) {
     constexpr int per_graph = 8;
-    const int D = robot_dof(robot);
-    const size_t lds = (size_t)lds_scene_f4 * sizeof(float4) + kMaxTaskWaves * kWave * sizeof(uint64_t) +
-                       ((size_t)(max_ext + 5) * D + 4 * kMaxTaskWaves + 4) * sizeof(float);
+    const size_t lds = plan_lds_bytes(robot_dof(robot), lds_scene_f4, max_ext, 1);   // task area 0 only
     const long long tasks = (long long)P * max_nnc;
     if (tasks > INT32_MAX) throw HipError{hipErrorInvalidValue, "run_round_graph: 2^31 tasks per round"};
     hipGraph_t graph = nullptr;
@@ -919,6 +958,7 @@ void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, Plan
 static std::atomic<int> g_waves_per_cu[4] = {{0}, {0}, {0}, {0}};
 static std::atomic<int> g_task_waves[4] = {{0}, {0}, {0}, {0}};
 static std::atomic<int> g_batch_waves[4] = {{0}, {0}, {0}, {0}};
+static std::atomic<int> g_crowd_above[4] = {{0}, {0}, {0}, {0}};
 
 static int env_int(const char *name, int lo, int hi) {
     const char *e = std::getenv(name);
@@ -947,8 +987,27 @@ int plan_waves_per_cu(int robot) {
 }
 
 int plan_tail_problems(int robot, int max_nnc) {
-    // the throughput grid holds every open task of this many problems at once
+    // a grid of one-wave tasks holds every slot of this many problems at once
     return std::max(1, num_cus() * plan_waves_per_cu(robot) / std::max(1, max_nnc));
+}
+
+// The crowd threshold as set (prrtc_set_tuning, else PRRTC_CROWD_ABOVE):
+// > 0 a count, < 0 off, 0 the default.
+static int crowd_setting(int robot) {
+    static const int env = [] {
+        const char *e = std::getenv("PRRTC_CROWD_ABOVE");
+        return e ? std::atoi(e) : 0;
+    }();
+    const int v = robot >= 0 && robot < 4 ? g_crowd_above[robot].load(std::memory_order_relaxed) : 0;
+    return v != 0 ? v : env;
+}
+
+int plan_crowd_above(int robot, int max_nnc) {
+    // by default a crowd is more problems than the resident planner waves
+    // (kResidentPlanWavesPerCu per CU) can serve with one slot each
+    const int v = crowd_setting(robot);
+    if (v < 0) return 0;
+    return v > 0 ? v : std::max(1, num_cus() * kResidentPlanWavesPerCu / std::max(1, max_nnc));
 }
 
 int plan_grid(int robot, int waves, long long slots) {
@@ -1020,7 +1079,11 @@ static void start_group(GroupRun &gr, int path_cap) {
     for (int w = 0; w < words; w++) hm[w] = hm[words + w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
-    // --- upload (descs + control in one copy; scenes and init rows per problem)
+    hc->prio_policy = kPrioOldest;
+    hc->crowd_above = P > 1 ? plan_crowd_above(robot, hc->max_nnc) : 0;
+    // --- upload (descs + control in one copy; scenes and init rows per
+    // problem): the control block is complete before the copy is queued
+    // (the copy reads the pinned staging asynchronously)
     const hipStream_t s = ws.stream;
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
     for (int i = 0; i < P; i++)
@@ -1031,7 +1094,6 @@ static void start_group(GroupRun &gr, int path_cap) {
     long long total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
     const int waves = P == 1 ? plan_task_waves(robot) : plan_batch_waves(robot);
-    hc->prio_policy = kPrioOldest;
     const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
     PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
@@ -1203,6 +1265,7 @@ int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
         g_waves_per_cu[r].store(t->waves_per_cu, std::memory_order_relaxed);
         g_task_waves[r].store(t->waves_per_task, std::memory_order_relaxed);
         g_batch_waves[r].store(t->batch_waves_per_task, std::memory_order_relaxed);
+        g_crowd_above[r].store(t->crowd_above, std::memory_order_relaxed);
     }
     return PRRTC_OK;
 }
@@ -1215,6 +1278,7 @@ int prrtc_get_tuning(int robot, prrtc_tuning *t) {
     t->waves_per_cu = plan_waves_per_cu(robot);
     t->waves_per_task = plan_task_waves(robot);
     t->batch_waves_per_task = plan_batch_waves(robot);
+    t->crowd_above = crowd_setting(robot);
     return PRRTC_OK;
 }
 

@@ -86,7 +86,9 @@ struct PlanControl {
     // without the bit are served first; null: no preference
     unsigned long long *aged_mask;
     int young_rounds;
-    int pad2;
+    // more running problems than this: a workgroup claims one slot per wave
+    // and each wave runs its own (0: always one slot on all waves)
+    int crowd_above;
     // the running problem furthest along, (round << 32) | p, or 0: its open
     // slots are claimed first and its tasks run at raised wave priority, so
     // the longest problem of a batch is not slowed by the crowd
@@ -271,14 +273,46 @@ __device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int word
 // shuffles, then across waves through `red` (LDS, 2 * kMaxTaskWaves words).
 // `tree` and `cap` keep every column 16-byte aligned (cap % 4 == 0).  Every
 // thread of the workgroup calls it with the same arguments.
+// Floats of one task's LDS area: chain rows (1 + max_ext configurations),
+// 4 uniform vectors, and the cross-wave reduction buffers (2 x 2 x
+// kMaxTaskWaves words).  The leader's decision takes kCtlInts words.
+__host__ __device__ constexpr int task_area_floats(int D, int max_ext) {
+    return (max_ext + 5) * D + 4 * kMaxTaskWaves;
+}
+constexpr int kCtlInts = 8;
+
+// The threads that run one planner task: the whole workgroup (its waves
+// share the work and meet at workgroup barriers) or a single wave on its
+// own (then no workgroup barrier may be used: the other waves run other
+// tasks).  `waves` is uniform over the workgroup.
+struct Team {
+    int waves;   // 1..kMaxTaskWaves
+    int wave;    // this wave's index in the team
+    int nt;      // threads: kWave * waves
+    int tid;     // this thread's index in the team
+    __device__ __forceinline__ void sync() const {
+        if (waves > 1) {
+            __syncthreads();
+        } else {   // one wave: order its LDS accesses, no workgroup barrier
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+};
+__device__ __forceinline__ Team team_workgroup() {
+    return Team{(int)(blockDim.x / kWave), wave_id(), (int)blockDim.x, (int)threadIdx.x};
+}
+__device__ __forceinline__ Team team_wave() { return Team{1, 0, kWave, lane_id()}; }
+
 template <int D>
 __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq,
-                                       float *red) {
+                                       float *red, const Team &tm) {
 #pragma clang fp contract(off)
     float best = __builtin_huge_valf();
     int idx = 0x7fffffff;
     const int lane = lane_id();
-    const int W = blockDim.x / kWave, wv = wave_id();
+    const int W = tm.waves, wv = tm.wave;
     // kScanChunk-node chunks, chunk k to wave k % W: lane l reads nodes
     // c + 256u + 4l .. +3 (u = 0..3) of every column as float4, so 4*D 16-byte
     // loads are in flight per lane.  The last chunk is read whole too (the
@@ -324,7 +358,7 @@ __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const 
             red[2 * wv] = best;
             red[2 * wv + 1] = __int_as_float(idx);
         }
-        __syncthreads();
+        tm.sync();
         best = red[0];
         idx = __float_as_int(red[1]);
         for (int w = 1; w < W; w++) {
@@ -332,7 +366,7 @@ __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const 
             const int oi = __float_as_int(red[2 * w + 1]);
             if (ob < best || (ob == best && oi < idx)) { best = ob; idx = oi; }
         }
-        __syncthreads();   // `red` may be written again right after
+        tm.sync();   // `red` may be written again right after
     }
     best_sq = best;
     return idx;

@@ -52,6 +52,9 @@ constexpr int kMaxWavesPerCu = 32;
 constexpr int kDefaultTaskWaves = 4;
 constexpr int kDefaultBatchWaves = 4;
 constexpr int kDefaultWavesPerCu = 8;
+// planner waves a CU holds at once: the real-robot k_plan kernels need
+// 256 VGPRs plus AGPRs, one wave per SIMD
+constexpr int kResidentPlanWavesPerCu = 4;
 int plan_task_waves(int robot);
 int plan_batch_waves(int robot);
 int plan_waves_per_cu(int robot);
@@ -59,9 +62,13 @@ int plan_grid(int robot, int waves, long long slots);
 void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
                      int P, int max_nnc, int waves, int lds_scene_f4, int max_ext, const unsigned long long *d_run,
                      unsigned long long *h_run, int words, long long max_rounds, hipStream_t s);
-// Resident problems up to which a server slice runs the single-solve shape:
-// as many as a one-wave grid holds with all their tasks in flight.
+// Resident problems up to which a server slice runs the single-solve shape
+// (waves_per_task) instead of the batch shape: as many as waves_per_cu x
+// CUs one-wave tasks hold with every slot in flight.
 int plan_tail_problems(int robot, int max_nnc);
+// Running problems above which a workgroup claims one slot per wave
+// (PlanControl::crowd_above; PRRTC_CROWD_ABOVE, < 0 turns it off).
+int plan_crowd_above(int robot, int max_nnc);
 
 // Tree roots and Halton states of the problems listed in d_list (table
 // entries), or of entries 0..n-1 when d_list is null.

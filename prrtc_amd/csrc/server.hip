@@ -305,6 +305,7 @@ void ServerImpl::launch(RobotClass &c) {
     hc.slice_ticks = (unsigned long long)cfg.slice_us * ticks_per_us;
     hc.mask_words = words;
     hc.prio_policy = kPrioYoungest;
+    hc.crowd_above = plan_crowd_above(c.robot, max_nnc);
     hc.run_mask = c.d_mask;
     hc.open_mask = c.d_mask + W;
     if (cfg.young_rounds > 0) {

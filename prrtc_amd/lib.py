@@ -67,7 +67,8 @@ class CBatchStats(C.Structure):
 
 
 class CTuning(C.Structure):
-    _fields_ = [("waves_per_cu", C.c_int32), ("waves_per_task", C.c_int32), ("batch_waves_per_task", C.c_int32)]
+    _fields_ = [("waves_per_cu", C.c_int32), ("waves_per_task", C.c_int32), ("batch_waves_per_task", C.c_int32),
+                ("crowd_above", C.c_int32)]
 
 
 class CServerConfig(C.Structure):
@@ -193,10 +194,11 @@ def settings_default() -> dict:
     return {name: getattr(st, name) for name, _ in CSettings._fields_}
 
 
-def set_tuning(robot, waves_per_cu: int = 0, waves_per_task: int = 0, batch_waves_per_task: int = 0) -> None:
+def set_tuning(robot, waves_per_cu: int = 0, waves_per_task: int = 0, batch_waves_per_task: int = 0,
+               crowd_above: int = 0) -> None:
     """Launch shapes of the persistent planner for `robot` (every robot when
     robot is None); 0 = default.  See prrtc_tuning in include/prrtc.h."""
-    t = CTuning(int(waves_per_cu), int(waves_per_task), int(batch_waves_per_task))
+    t = CTuning(int(waves_per_cu), int(waves_per_task), int(batch_waves_per_task), int(crowd_above))
     _check(load().prrtc_set_tuning(-1 if robot is None else robot_id(robot), C.byref(t)), "prrtc_set_tuning")
 
 

@@ -84,6 +84,8 @@ def test_invalid_arguments_fail_without_gpu():
     from prrtc_amd.lib import CTuning
     assert L.prrtc_set_tuning(1, C.byref(CTuning(0, 9, 0))) == -1
     assert L.prrtc_set_tuning(7, C.byref(CTuning(0, 0, 0))) == -1
+    t = CTuning()
+    assert L.prrtc_get_tuning(2, C.byref(t)) == 0 and t.crowd_above == 0   # 0: the default threshold
 
 
 def test_server_rejects_bad_config_without_gpu():

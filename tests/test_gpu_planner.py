@@ -199,7 +199,10 @@ def test_forced_failures(gpu, oracle_robots):
 def test_task_waves_same_results(waves, gpu, oracle_robots):
     """A task spread over several waves (shared nearest-node scans, obstacle
     tables split between the waves of each validation pass) returns bitwise
-    the single-wave result, alone and inside a one-wave batch."""
+    the single-wave result, alone and inside a one-wave batch; so does a
+    batch of `waves`-wave workgroups with crowd claims forced on (each wave
+    runs its own slot while more than one problem runs, the waves share a
+    slot for the last problem)."""
     from prrtc_amd import scenes as S
     from prrtc_amd.robots import fixed_spheres, limits
     cases = []
@@ -223,12 +226,15 @@ def test_task_waves_same_results(waves, gpu, oracle_robots):
         gpu.set_tuning(None, waves_per_task=waves, batch_waves_per_task=1)
         assert gpu.get_tuning("panda")["waves_per_task"] == waves
         got = [gpu.solve(*c) for c in cases]
-        batch, bpaths, _ = gpu.solve_batch([dict(robot=c[0], start=c[1], goals=c[2], scene=c[3]) for c in cases],
-                                           [c[4] for c in cases])
+        problems = [dict(robot=c[0], start=c[1], goals=c[2], scene=c[3]) for c in cases]
+        batch, bpaths, _ = gpu.solve_batch(problems, [c[4] for c in cases])
+        gpu.set_tuning(None, waves_per_task=waves, batch_waves_per_task=waves, crowd_above=1)
+        assert gpu.get_tuning("fetch")["crowd_above"] == 1
+        crowd, cpaths, _ = gpu.solve_batch(problems, [c[4] for c in cases])
     finally:
-        gpu.set_tuning(None, 0, 0, 0)
-    for (robot, *_), (r1, p1), (r2, p2), rb, pb in zip(cases, ref, got, batch, bpaths):
+        gpu.set_tuning(None, 0, 0, 0, 0)
+    for (robot, *_), (r1, p1), (r2, p2), rb, pb, rc, pc in zip(cases, ref, got, batch, bpaths, crowd, cpaths):
         print(robot, r1["solved"], r1["iters"], r1["start_tree_size"], r1["goal_tree_size"], flush=True)
         for k in ("solved", "iters", "start_tree_size", "goal_tree_size", "path_length", "edge_checks"):
-            assert r1[k] == r2[k] == rb[k], (robot, k, r1[k], r2[k], rb[k])
-        assert p1.tobytes() == p2.tobytes() == pb.tobytes(), robot
+            assert r1[k] == r2[k] == rb[k] == rc[k], (robot, k, r1[k], r2[k], rb[k], rc[k])
+        assert p1.tobytes() == p2.tobytes() == pb.tobytes() == pc.tobytes(), robot

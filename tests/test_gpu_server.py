@@ -44,16 +44,22 @@ def assert_same(r, path, r1, path1, what):
     assert path.tobytes() == path1.tobytes(), what
 
 
-@pytest.mark.parametrize("young_rounds", [0, -1, 2])
-def test_server_same_as_alone(gpu, oracle_robots, young_rounds):
+@pytest.mark.parametrize("young_rounds,crowd", [(0, False), (-1, False), (2, False), (0, True)])
+def test_server_same_as_alone(gpu, oracle_robots, young_rounds, crowd):
     """Claim order (young requests first for 64 or 2 rounds, or plain
-    round-robin) changes no result."""
+    round-robin) and crowd claims (one slot per wave while more than one
+    request runs) change no result."""
     reqs = mixed_requests(oracle_robots, 12)
     alone = [gpu.solve(p["robot"], p["start"], p["goals"], p["scene"], st) for p, st in reqs]
-    with gpu.Server(slice_us=150, young_rounds=young_rounds) as srv:
-        tickets = [srv.submit(p, st) for p, st in reqs]
-        out = [srv.poll(t) for t in tickets]
-        stats = srv.stats()
+    try:
+        if crowd:
+            gpu.set_tuning(None, batch_waves_per_task=4, crowd_above=1)
+        with gpu.Server(slice_us=150, young_rounds=young_rounds) as srv:
+            tickets = [srv.submit(p, st) for p, st in reqs]
+            out = [srv.poll(t) for t in tickets]
+            stats = srv.stats()
+    finally:
+        gpu.set_tuning(None, 0, 0, 0, 0)
     multi = 0
     for (p, _), (r1, path1), (r, path, tm), in zip(reqs, alone, out):
         print(p["robot"], r["solved"], r["iters"], tm, flush=True)
