@@ -92,7 +92,10 @@ int prrtc_robot_dof(int robot);
 int prrtc_robot_num_spheres(int robot);
 
 /* ---- scenes ------------------------------------------------------------ */
-/* Packed device form of a scene (float4-aligned tables, counts in a header). */
+/* Packed device form of a scene (float4-aligned tables, counts in a header).
+ * Each cuboid also carries a cull radius computed here (its padded
+ * half-diagonal), so device buffers for the *_async entry points must be
+ * written by prrtc_scene_pack, not assembled by hand. */
 size_t prrtc_scene_packed_floats(const prrtc_scene *scene);
 int prrtc_scene_pack(const prrtc_scene *scene, float *out, size_t cap_floats);
 

@@ -28,8 +28,8 @@ constexpr size_t kMaxLdsScene = 64 * 1024;
 //   spheres  : 1 float4 each   (x, y, z, r)
 //   capsules : 2 float4 each   (x1, y1, z1, r) (xv, yv, zv, rdv)
 //   z-caps   : 2 float4 each
-//   cuboids  : 4 float4 each   (x, y, z, r1) (a1, r2) (a2, r3) (a3, 0)
-//   z-cuboids: 4 float4 each
+//   cuboids  : 4 float4 each   (x, y, z, r1) (a1, r2) (a2, r3) (a3, cull)
+//   z-cuboids: 4 float4 each   (cull: padded half-diagonal, set by the host pack)
 struct SceneHeader {
     int32_t n_sph, n_cap, n_zcap, n_cub, n_zcub, n_f4, pad0, pad1;
 };
@@ -122,7 +122,24 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 // verdict bit for bit in its "fused" mode.
 __device__ __forceinline__ float sq3(float dx, float dy, float dz) { return fmaf(dx, dx, fmaf(dy, dy, dz * dz)); }
 
-template <int N, bool EARLY>
+// Obstacle table read.  LOCAL: the tables are known to be staged in LDS, so
+// the generic pointer is narrowed to an LDS address (its low 32 bits) and
+// read with an LDS instruction, where the compiler cannot see the pointer's
+// origin (a caller that is not inlined into the kernel) and would otherwise
+// emit a flat load.
+template <bool LOCAL>
+__device__ __forceinline__ float4 ld_obs(const float4 *p) {
+    if constexpr (LOCAL) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        typedef const __attribute__((address_space(3))) f4v local_f4v;
+        const f4v v = *reinterpret_cast<local_f4v *>((uint32_t)(uintptr_t)p);
+        return make_float4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+
+template <int N, bool EARLY, bool LOCAL = false>
 __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
                                           const float (&z)[N], const float (&r)[N], bool (&hit)[N]) {
 #pragma clang fp contract(off)
@@ -144,7 +161,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         for (int n = 0; n < N; n++) hit[n] = m[n] < 0.0f;
     };
     for (int i = 0; i < o.n_sph; i++) {                       // utils.cuh:58-81
-        const float4 s = o.sph[i];
+        const float4 s = ld_obs<LOCAL>(o.sph + i);
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float rs = s.w + r[n];
@@ -153,10 +170,18 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
     if (EARLY && o.n_sph && done()) return finish();
+    // Capsules are culled like cuboids (below) against the ball about the
+    // segment's midpoint that holds the capsule: radius |v|/2 + r, padded by
+    // 2^-16 relative and 1 mm absolute.  The pad covers the rounding of
+    // sqrtf and of both tests many times over.
     for (int i = 0; i < o.n_cap; i++) {                       // utils.cuh:113-126
-        const float4 p = o.cap[2 * i], v = o.cap[2 * i + 1];
+        const float4 p = ld_obs<LOCAL>(o.cap + 2 * i), v = ld_obs<LOCAL>(o.cap + 2 * i + 1);
+        const float mx = fmaf(0.5f, v.x, p.x), my = fmaf(0.5f, v.y, p.y), mz = fmaf(0.5f, v.z, p.z);
+        const float cull = fmaf(0.5f * sqrtf(sq3(v.x, v.y, v.z)) + p.w, 1.0f + 0x1p-16f, 1e-3f);
 #pragma unroll
         for (int n = 0; n < N; n++) {
+            const float rb = r[n] + cull;
+            if (ballot(!(sq3(x[n] - mx, y[n] - my, z[n] - mz) > rb * rb)) == 0) continue;
             const float dot = fmaf(x[n] - p.x, v.x, fmaf(y[n] - p.y, v.y, (z[n] - p.z) * v.z));
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
             const float rs = r[n] + p.w;
@@ -167,7 +192,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
     }
     if (EARLY && o.n_cap && done()) return finish();
     for (int i = 0; i < o.n_zcap; i++) {                      // utils.cuh:135-148
-        const float4 p = o.zcap[2 * i], v = o.zcap[2 * i + 1];
+        const float4 p = ld_obs<LOCAL>(o.zcap + 2 * i), v = ld_obs<LOCAL>(o.zcap + 2 * i + 1);
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float dot = (z[n] - p.z) * v.z;
@@ -179,11 +204,19 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
     if (EARLY && o.n_zcap && done()) return finish();
+    // Cuboids carry a cull radius (a3.w): the box's half-diagonal with a 1 mm
+    // pad, from the host pack.  When no lane's sphere comes within r + cull
+    // of the centre, the box test is positive by at least that pad on every
+    // lane and cannot lower m below 0, so the wave skips it (the test itself,
+    // where it runs, is unchanged).
     for (int i = 0; i < o.n_cub; i++) {                       // utils.cuh:158-175
-        const float4 c = o.cub[4 * i], a1 = o.cub[4 * i + 1], a2 = o.cub[4 * i + 2], a3 = o.cub[4 * i + 3];
+        const float4 c = ld_obs<LOCAL>(o.cub + 4 * i), a1 = ld_obs<LOCAL>(o.cub + 4 * i + 1),
+                     a2 = ld_obs<LOCAL>(o.cub + 4 * i + 2), a3 = ld_obs<LOCAL>(o.cub + 4 * i + 3);
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
+            const float rb = r[n] + a3.w;
+            if (ballot(!(sq3(xs, ys, zs) > rb * rb)) == 0) continue;
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, fmaf(a1.y, ys, a1.z * zs))) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, fmaf(a2.y, ys, a2.z * zs))) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(fmaf(a3.x, xs, fmaf(a3.y, ys, a3.z * zs))) - a2.w);
@@ -193,10 +226,14 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
     }
     if (EARLY && o.n_cub && done()) return finish();
     for (int i = 0; i < o.n_zcub; i++) {                      // utils.cuh:184-201
-        const float4 c = o.zcub[4 * i], a1 = o.zcub[4 * i + 1], a2 = o.zcub[4 * i + 2];
+        const float4 c = ld_obs<LOCAL>(o.zcub + 4 * i), a1 = ld_obs<LOCAL>(o.zcub + 4 * i + 1),
+                     a2 = ld_obs<LOCAL>(o.zcub + 4 * i + 2);
+        const float cull = ld_obs<LOCAL>(o.zcub + 4 * i + 3).w;
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
+            const float rb = r[n] + cull;
+            if (ballot(!(sq3(xs, ys, zs) > rb * rb)) == 0) continue;
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, a1.y * ys)) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, a2.y * ys)) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(zs) - a2.w);

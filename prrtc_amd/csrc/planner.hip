@@ -44,8 +44,25 @@ __device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int 
 // the waves' collision flags are combined after the pass.  A gate hits iff
 // some obstacle hits it, whichever wave tests that obstacle, so the verdicts
 // are those of a single wave that sees every obstacle.
+//
+// The waves also share their early exits: in each pass a wave that finds a
+// collision posts its lowest colliding key to an LDS word (atomic min), and
+// at every check-group boundary each wave retires its lanes whose key is at
+// or above the posted one.  Such lanes cannot lower the pass's answer (the
+// lowest colliding key over all waves, or "any collision" on one edge), so
+// a collision found on any wave ends the pass on all of them without a
+// barrier.  The word is chosen by pass parity and reset a pass ahead, so a
+// wave never reads a previous pass's key.
+// The two words live in static LDS, so every access is a plain LDS
+// instruction; only a task of the whole workgroup uses them.
+__device__ __forceinline__ int *task_stop_words() {
+    __shared__ int words[2];
+    return words;
+}
+
 struct TaskCtx : CheckCtx {
     uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
+    int pass;         // index of the current pass within the task
     int waves;        // waves of the task: the whole workgroup, or 1
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
         const int W = waves;
@@ -57,6 +74,25 @@ struct TaskCtx : CheckCtx {
         for (int w = 0; w < W; w++) g |= gbuf[w * kWave + lane];
         __syncthreads();   // gbuf is written again by the next pass
         gate = g;
+    }
+    // The obstacle tables of a task are always staged in LDS.
+    template <int N, bool EARLY>
+    __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                              const float (&r)[N], bool (&hit)[N]) const {
+        prrtc::env_multi<N, EARLY, true>(obs, x, y, z, r, hit);
+    }
+    // CheckCtx::stop in path mode (a planner task is never in pose mode),
+    // with the stop key shared across waves.
+    __device__ __forceinline__ bool stop() {
+        const uint64_t cm = ballot(coll);
+        int sk = cm ? __builtin_amdgcn_readlane(key, __builtin_ctzll(cm)) : 0x7fffffff;
+        if (waves > 1) {
+            int *w = task_stop_words() + (pass & 1);
+            if (cm && lane_id() == 0) __hip_atomic_fetch_min(w, sk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            sk = min(sk, __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        }
+        dead = dead || key >= sk;
+        return ballot(!coll && !dead) == 0;
     }
 };
 
@@ -72,9 +108,13 @@ struct TaskCtx : CheckCtx {
 // node, c_{i+1} = c_i + vec; uv holds the nearest node, the steered step and
 // the connect step; red and wbuf are the cross-wave reduction buffers.
 template <class Robot>
-__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx, float *chain, float *uv,
+__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx &ctx_in, float *chain, float *uv,
                           float *red, int *wbuf, const Team &tm) {
 #pragma clang fp contract(off)
+    // a private copy: its fields stay in registers across the check sequence
+    // (through the caller's reference they would be reloaded after every
+    // store to the pass flags)
+    TaskCtx ctx = ctx_in;
     constexpr int D = Robot::DOF;
     const int tid = tm.tid, NT = tm.nt;
     const int lane = lane_id();
@@ -86,6 +126,8 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
     float *stage = d.stage + (size_t)s * (d.max_ext + 1) * D;
     float *nnv = uv, *delta = uv + D, *step = uv + 2 * D, *vecv = uv + 3 * D;
     int edges = 0;
+    int *stopw = task_stop_words();
+    if (tid == 0 && tm.waves > 1) stopw[0] = stopw[1] = 0x7fffffff;   // read only after the next barrier
 
     // tree choice and Halton sample (uniform across the workgroup)
     const int t = choose_tree(d, s, iter, d.tprev[s], n0, n1);
@@ -158,6 +200,10 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, TaskCtx &ctx,
             ctx.coll = false;
             ctx.dead = dead;
             ctx.key = key;
+            ctx.pass = npass;
+            // the other parity's word is next read in the following pass,
+            // after this pass's closing barrier
+            if (tid == 0 && tm.waves > 1) stopw[(npass + 1) & 1] = 0x7fffffff;
             Robot::check(q, ctx);
             const uint64_t cm = ballot(ctx.coll);
             if (phase == 1) {

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 
 #include "host_common.h"
@@ -151,6 +152,18 @@ bool scene_valid(const prrtc_scene *s) {
     return true;
 }
 
+// Cull radius of a cuboid with half-extents h: its half-diagonal |h| (the box
+// lies inside the ball of that radius about its centre), padded by 2^-16
+// relative and 1 mm absolute.  A robot sphere (radius r) whose centre is
+// farther than r + cull from the box centre is at least 1 mm from the box,
+// so the box test's value is positive by a margin far above the rounding
+// of either computation: skipping the test cannot change a verdict.  A
+// non-finite extent gives a non-finite radius, which never culls.
+static float cuboid_cull_radius(const float *h) {
+    const double R = std::sqrt((double)h[0] * h[0] + (double)h[1] * h[1] + (double)h[2] * h[2]);
+    return (float)(R * (1.0 + 1.0 / 65536.0) + 1e-3);
+}
+
 size_t scene_packed_floats(const prrtc_scene *s) {
     return 8 + 4 * (size_t)scene_payload_f4(s->n_spheres, s->n_capsules, s->n_z_capsules, s->n_cuboids,
                                             s->n_z_cuboids);
@@ -175,7 +188,7 @@ void scene_pack(const prrtc_scene *s, float *out) {
         for (int i = 0; i < n; i++, o += 16) {
             const float *r = c + 15 * i;  // x y z a1 a2 a3 r1 r2 r3
             const float v[16] = {r[0], r[1], r[2], r[12], r[3], r[4], r[5], r[13],
-                                 r[6], r[7], r[8], r[14], r[9], r[10], r[11], 0.0f};
+                                 r[6], r[7], r[8], r[14], r[9], r[10], r[11], cuboid_cull_radius(r + 12)};
             std::memcpy(o, v, sizeof v);
         }
     };

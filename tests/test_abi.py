@@ -57,7 +57,9 @@ def test_scene_pack_layout():
     assert len(p) == 8 + 4 * 7
     np.testing.assert_array_equal(p[8:12], [1, 2, 3, 0.5])
     np.testing.assert_allclose(p[12:20], [0, 0, 0, 0.1, 0, 0, 2, 0.25])
-    np.testing.assert_allclose(p[20:36], [1, 1, 1, 0.1, 1, 0, 0, 0.2, 0, 1, 0, 0.3, 0, 0, 1, 0], atol=1e-7)
+    # the last float of a cuboid is its cull radius: half-diagonal padded by 2^-16 and 1 mm
+    cull = np.sqrt(0.1 ** 2 + 0.2 ** 2 + 0.3 ** 2) * (1 + 2.0 ** -16) + 1e-3
+    np.testing.assert_allclose(p[20:36], [1, 1, 1, 0.1, 1, 0, 0, 0.2, 0, 1, 0, 0.3, 0, 0, 1, cull], atol=1e-7)
 
 
 def test_invalid_arguments_fail_without_gpu():
