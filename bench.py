@@ -190,10 +190,15 @@ def main():
     achieved = flops / (roof_ms / 1e3)
     log(f"free set: {n_roof} edges, {roof_ms:.3f} ms, {per_pose:.0f} flops/pose, {achieved / 1e12:.2f} TFLOP/s")
     traffic = None
+    valu = None   # hardware VALU counters of the same kernel from the committed PMC summary
     tpath = os.path.join(ROOT, "profiles", "validate_edges_panda_free_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        if tj.get("valu"):
+            valu = {k: tj["valu"][k] for k in ("valu_busy", "hw_fp32_tflops", "valu_insts") if k in tj["valu"]}
+            valu["source"] = "rocprofv3 --pmc, profiles/validate_edges_panda_c1.md"
 
     out = {
         "metric": "plans/sec batched vs one-per-launch; p50/p99 solve ms; edge checks/sec",
@@ -218,7 +223,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_PACKED,
                      "frac_of_nonpacked_peak": achieved / PEAK_FP32, "traffic": traffic,
                      "kernel": "k_validate_edges<PandaRobot, true>", "kernel_ms": roof_ms,
-                     "flops_per_pose": per_pose, "poses_per_launch": n_roof * g},
+                     "flops_per_pose": per_pose, "poses_per_launch": n_roof * g, "valu_counters": valu},
     }
     paths_to_check = []   # (robot, scene, path, g)
 

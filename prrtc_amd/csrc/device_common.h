@@ -271,6 +271,9 @@ struct CheckCtx {
 
     __device__ __forceinline__ bool live() const { return !coll && !dead; }
     __device__ __forceinline__ bool any(bool p) const { return ballot(p) != 0; }
+    // Self-collision group j (numbered in check order) is this wave's to
+    // test; a lone wave tests every group.
+    __device__ __forceinline__ bool mine(int) const { return true; }
     template <int N, bool EARLY>
     __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
                                               const float (&r)[N], bool (&hit)[N]) const {

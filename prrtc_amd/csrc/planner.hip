@@ -53,6 +53,11 @@ __device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int 
 // a collision found on any wave ends the pass on all of them without a
 // barrier.  The word is chosen by pass parity and reset a pass ahead, so a
 // wave never reads a previous pass's key.
+//
+// Self-collision groups, which need no obstacles, are dealt out instead:
+// group j goes to wave j mod W (`mine`), and a hit on any wave ends the pass
+// on all of them through the same word.  Every group is still tested by one
+// wave for every live lane, so the verdicts are unchanged.
 // The two words live in static LDS, so every access is a plain LDS
 // instruction; only a task of the whole workgroup uses them.
 __device__ __forceinline__ int *task_stop_words() {
@@ -64,6 +69,7 @@ struct TaskCtx : CheckCtx {
     uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
     int pass;         // index of the current pass within the task
     int waves;        // waves of the task: the whole workgroup, or 1
+    unsigned mine12;  // bit i: self groups j with j mod 12 == i are this wave's (12 = lcm of 1..4)
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
         const int W = waves;
         if (W == 1) return;
@@ -75,6 +81,8 @@ struct TaskCtx : CheckCtx {
         __syncthreads();   // gbuf is written again by the next pass
         gate = g;
     }
+    static_assert(kMaxTaskWaves <= 4, "the 12-group pattern of mine12 needs 12 % waves == 0");
+    __device__ __forceinline__ bool mine(int j) const { return (mine12 >> (j % 12)) & 1u; }
     // The obstacle tables of a task are always staged in LDS.
     template <int N, bool EARLY>
     __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
@@ -115,6 +123,12 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx
     // (through the caller's reference they would be reloaded after every
     // store to the pass flags)
     TaskCtx ctx = ctx_in;
+    {
+        unsigned m = 0;
+        for (int i = 0; i < 12; i++) m |= (unsigned)(i % tm.waves == tm.wave) << i;
+        // uniform over the wave: the group branches stay scalar branches
+        ctx.mine12 = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
+    }
     constexpr int D = Robot::DOF;
     const int tid = tm.tid, NT = tm.nt;
     const int lane = lane_id();
