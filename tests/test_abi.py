@@ -98,3 +98,18 @@ def test_server_rejects_bad_config_without_gpu():
     assert L.prrtc_server_create(C.byref(cfg), C.byref(h)) == -1
     assert not h.value
     assert L.prrtc_server_create(None, None) == -1
+
+
+def test_cuboid_cull_radius():
+    """The pack's cull radius bounds the box: at least its half-diagonal,
+    at most that plus the stated pads (2^-16 relative, 1 mm)."""
+    import prrtc_amd
+    from prrtc_amd import scenes as S
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        h = rng.uniform(0.001, 2.0, 3).astype(np.float32)
+        sc = S.Scene.build(cuboids=[S.cuboid(rng.uniform(-1, 1, 3), rng.uniform(-3, 3, 3), h)])
+        cull = float(prrtc_amd.pack_scene(sc)[8 + 15])
+        R = float(np.linalg.norm(h.astype(np.float64)))
+        assert R + 1e-3 <= cull * (1 + 1e-7)
+        assert cull <= R * (1 + 2.0 ** -16) + 1e-3 + 1e-6

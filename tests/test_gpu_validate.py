@@ -89,3 +89,43 @@ def test_nonfinite_rejected(gpu):
     e[1, 1, 3] = np.nan
     with pytest.raises(gpu.PrrtcError):
         gpu.validate_edges("panda", S.scene_spheres(0), e, 32)
+
+
+def test_cull_boundary(gpu, oracle_robots):
+    """Poses a fraction of a millimetre to 2 mm from contact where the
+    bounding balls of the culls are tight: beyond a cuboid's corners along
+    its diagonals and beyond a capsule's ends along its axis.  The culled
+    kernel must give the oracle's verdict on every one (none is within
+    1e-6 m of contact)."""
+    from prrtc_amd import scenes as S
+    rng = np.random.default_rng(7)
+    c = np.array([0.3, -0.2, 0.5], np.float32)
+    h = np.array([0.4, 0.7, 0.25], np.float32)
+    cub = S.cuboid(c, (0.3, -0.5, 1.1), h)
+    R = cub[3:12].reshape(3, 3).T          # columns are the box axes
+    p1, p2 = np.array([-2.0, 1.5, 0.2], np.float32), np.array([-1.0, 2.5, 1.4], np.float32)
+    cap = S.capsule_endpoints(p1, p2, 0.3)
+    sc = S.Scene.build(cuboids=[cub], capsules=[cap])
+    deltas = np.array([-2e-3, -5e-4, -1e-4, 1e-4, 5e-4, 2e-3], np.float32)
+    pts = []
+    for sx in (-1, 1):
+        for sy in (-1, 1):
+            for sz in (-1, 1):
+                corner = c + R @ (h * np.array([sx, sy, sz], np.float32))
+                u = R @ (h * np.array([sx, sy, sz], np.float32))
+                u /= np.linalg.norm(u)
+                for d in deltas:
+                    pts.append(corner + u * (1.0 + d))                      # robot radius 1
+                    w = u + 0.05 * rng.normal(size=3)
+                    pts.append(corner + w / np.linalg.norm(w) * (1.0 + d))
+    axis = (p2 - p1) / np.linalg.norm(p2 - p1)
+    for end, s in ((p1, -1.0), (p2, 1.0)):
+        for d in deltas:
+            pts.append(end + s * axis * (1.3 + d))                            # robot 1 + capsule 0.3
+    poses = np.array(pts, np.float32)
+    R_or = oracle_robots["sphere"]
+    want, margin = R_or.check(sc, poses, margin=True)
+    got = gpu.check_poses("sphere", sc, poses)
+    assert np.all(margin >= 1e-6)
+    assert 0 < want.sum() < len(want)
+    assert np.array_equal(got, want), f"{(got != want).sum()} of {len(want)} verdicts differ"
