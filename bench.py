@@ -18,6 +18,8 @@ each step is one launch of the edge-validation kernel over all edges.
     batched      256 Fetch problems (8-64 mixed obstacles, g 64): plans/s of
                  one persistent launch vs one launch per problem, and of the
                  same batch as graph-replayed round grids
+    batched_per_robot  the same comparison for Panda and Baxter on the
+                 shelf problem sets of the mix (each class's settings)
     mix          128 Baxter + 256 Panda + 128 Fetch shelf problems (g 128,
                  dynamic domain, balance 1) in one call
     serving      Poisson request stream (Panda:Fetch:Baxter 2:1:1, fresh
@@ -84,6 +86,8 @@ def main():
     ap.add_argument("--plans", type=int, default=16, help="single-problem plans for p50/p99")
     ap.add_argument("--c0-seeds", type=int, default=128)
     ap.add_argument("--batch", type=int, default=256, help="Fetch problems of the batched vs plain comparison")
+    ap.add_argument("--robot-batch", type=int, default=96,
+                    help="Panda and Baxter shelf problems of their batched vs plain comparisons")
     ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -354,6 +358,40 @@ def main():
             f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x), identical={same}; "
             f"graph-replayed rounds {out['batched']['plans_per_s_graph_rounds']:.1f} plans/s, "
             f"identical={same_graph}")
+
+        # ------------------------------ batched vs plain, Panda and Baxter
+        # The same comparison on the shelf problem sets of the uneven mix,
+        # one class at a time, with each class's settings and tuned nnc: both
+        # forms solve the same connectable problems at the same granularity.
+        from prrtc_amd.autotune import SETTINGS as CLASS_SETTINGS, problems as class_problems
+        out["batched_per_robot"] = {}
+        for robot in ("panda", "baxter"):
+            rp = class_problems(robot, args.robot_batch)
+            cbase = CLASS_SETTINGS[robot]
+            pres, _, _ = P.solve_batch(rp, dict(cbase, num_new_configs=64, max_iters=2000))
+            rf = [p for p, r in zip(rp, pres) if r["solved"]]
+            rb = dict(cbase, num_new_configs=tuned.get("batch_nnc", {}).get(robot, 128), max_iters=20000)
+            rpl = dict(cbase, num_new_configs=tuned.get("plain_nnc", {}).get(robot, 128), max_iters=20000)
+            t0 = time.perf_counter()
+            rres, rpaths, _ = P.solve_batch(rf, rb)
+            tb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            rplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], rpl) for p in rf]
+            tp = time.perf_counter() - t0
+            nb_r = sum(r["solved"] for r in rres)
+            np_r = sum(r[0]["solved"] for r in rplain)
+            out["batched_per_robot"][robot] = {
+                "problems": len(rp), "feasible": len(rf), "solved_batched": nb_r, "solved_plain": np_r,
+                "plans_per_s_batched": nb_r / tb, "plans_per_s_plain": np_r / tp,
+                "speedup": (nb_r / tb) / max(1e-9, np_r / tp), "nnc_batched": rb["num_new_configs"],
+                "nnc_plain": rpl["num_new_configs"], "granularity": cbase["granularity"]}
+            log(f"{robot} batched {nb_r / tb:.1f} plans/s vs plain {np_r / tp:.1f} plans/s "
+                f"({(nb_r / tb) / max(1e-9, np_r / tp):.2f}x) on {len(rf)} connectable shelf problems")
+            g_r = cbase["granularity"]
+            for p, r, pa in zip(rf, rres, rpaths):
+                if r["solved"]:
+                    paths_to_check.append((robot, p["scene"], pa, g_r))
+
         for p, r, pa in zip(feas, bres, bpaths):
             if r["solved"]:
                 paths_to_check.append(("fetch", p["scene"], pa, 64))
