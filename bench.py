@@ -368,8 +368,8 @@ def main():
         for robot in ("panda", "baxter"):
             rp = class_problems(robot, args.robot_batch)
             cbase = CLASS_SETTINGS[robot]
-            pres, _, _ = P.solve_batch(rp, dict(cbase, num_new_configs=64, max_iters=2000))
-            rf = [p for p, r in zip(rp, pres) if r["solved"]]
+            rpres, _, _ = P.solve_batch(rp, dict(cbase, num_new_configs=64, max_iters=2000))
+            rf = [p for p, r in zip(rp, rpres) if r["solved"]]
             rb = dict(cbase, num_new_configs=tuned.get("batch_nnc", {}).get(robot, 128), max_iters=20000)
             rpl = dict(cbase, num_new_configs=tuned.get("plain_nnc", {}).get(robot, 128), max_iters=20000)
             t0 = time.perf_counter()
