@@ -352,6 +352,8 @@ def main():
                           "graph_rounds_wall_s": t_graph, "graph_rounds_kernel_ms": gstats["kernel_ns"] / 1e6,
                           "graph_rounds_identical": bool(same_graph),
                           "nnc_batched": nnc_b, "nnc_plain": nnc_p,
+                          "cost_p50": pct([r["cost"] for r in bres if r["solved"]], 50),
+                          "cost_p90": pct([r["cost"] for r in bres if r["solved"]], 90),
                           "same_settings_identical_results": bool(same),
                           "settings": "fetch, g 64, range 0.5, balance 2, dd off; nnc per mode from the autotuner"}
         log(f"batched {out['batched']['plans_per_s_batched']:.1f} plans/s vs plain "
@@ -489,17 +491,29 @@ def main():
             fr = model("fetch")
 
             def cpu_solve(p):
-                return fr.rrt_connect(p["scene"], p["start"], p["goals"][0], 64, 0.5, 5000, 1)[1].solved
+                res = fr.rrt_connect(p["scene"], p["start"], p["goals"][0], 64, 0.5, 5000, 1)[1]
+                return bool(res.solved), float(res.cost)
 
             t0 = time.perf_counter()
             with ThreadPoolExecutor(threads) as ex:
-                cpu_solved = sum(ex.map(cpu_solve, fprobs))
+                cpu_res = list(ex.map(cpu_solve, fprobs))
             dt = time.perf_counter() - t0
+            cpu_solved = sum(sv for sv, _ in cpu_res)
+            # path cost against the GPU planner's on the problems both solved
+            # (the same connectable set, in order)
+            both = [(g["cost"], c) for g, (sv, c) in zip(bres, cpu_res) if sv and g["solved"]]
             out["cpu_rrt_connect_c2"] = {"problems": len(fprobs), "solved": int(cpu_solved),
+                                         "cost_p50": pct([c for sv, c in cpu_res if sv], 50),
+                                         "cost_p90": pct([c for sv, c in cpu_res if sv], 90),
+                                         "gpu_over_cpu_cost_p50": pct([g / c for g, c in both if c > 0], 50),
+                                         "both_solved": len(both),
                                          "plans_per_s": cpu_solved / dt, "wall_s": dt, "cores": threads,
                                          "kind": "project CPU RRT-Connect (vamp unavailable offline)",
                                          "settings": "fetch, g 64, range 0.5, max 5000 samples per problem"}
-            log(f"cpu rrt-connect c2: {cpu_solved}/{len(fprobs)} solved in {dt:.1f} s on {threads} threads")
+            log(f"cpu rrt-connect c2: {cpu_solved}/{len(fprobs)} solved in {dt:.1f} s on {threads} threads; "
+                f"path cost p50 {out['cpu_rrt_connect_c2']['cost_p50']:.2f} vs GPU batch "
+                f"{out['batched']['cost_p50']:.2f} (GPU/CPU p50 {out['cpu_rrt_connect_c2']['gpu_over_cpu_cost_p50']:.2f} "
+                f"on {len(both)} problems both solved)")
         if "mix" in out:
             # the uneven mix through the CPU RRT-Connect (its own robot per
             # problem), bounded at 5000 samples per problem
