@@ -175,6 +175,15 @@ def main():
         nn_ms = time_launches(lambda: P.nearest_async(ndof, d_trees.data_ptr(), ncap, n_nodes, n_trees,
                                                       d_q.data_ptr(), 1, 4, d_idx.data_ptr(), d_d2.data_ptr(),
                                                       stream), args.steps, torch)
+        # then the planner-like case: 64 queries per tree (the queries of one
+        # tree share its lines through L2)
+        qpt = 64
+        d_q64 = torch.rand((n_trees, qpt, ndof), generator=gen, device=dev)
+        d_idx64 = torch.zeros(n_trees * qpt, dtype=torch.int32, device=dev)
+        d_d264 = torch.zeros(n_trees * qpt, dtype=torch.float32, device=dev)
+        nn64_ms = time_launches(lambda: P.nearest_async(ndof, d_trees.data_ptr(), ncap, n_nodes, n_trees,
+                                                        d_q64.data_ptr(), qpt, 4, d_idx64.data_ptr(),
+                                                        d_d264.data_ptr(), stream), args.steps, torch)
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)       # 256 MiB
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -182,6 +191,7 @@ def main():
         print(json.dumps({"profile_run": True, "validate_launches": args.steps + 1, "edges_per_launch": n_roof,
                           "kernel_ms": roof_ms, "nearest_launches": args.steps, "nearest_ms": nn_ms,
                           "nearest_bytes_per_launch": 4 * ndof * n_nodes * n_trees,
+                          "nearest64_launches": args.steps, "nearest64_ms": nn64_ms,
                           "copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}), flush=True)
         return
     rng = np.random.default_rng(5)

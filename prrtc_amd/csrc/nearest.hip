@@ -23,7 +23,19 @@ __global__ __launch_bounds__(kMaxTaskWaves * kWave) void k_nearest(const float *
                                                                     int n_queries, int *__restrict__ idx,
                                                                     float *__restrict__ dist2) {
     __shared__ float red[2 * kMaxTaskWaves];
-    for (int k = blockIdx.x; k < n_queries; k += gridDim.x) {
+    // Workgroups reach the XCDs round-robin (workgroup b runs on XCD b % 8)
+    // and each XCD has its own L2.  The queries are split into 8 contiguous
+    // ranges, one per XCD, so the queries of one tree (consecutive k) share
+    // that tree's lines in one L2 instead of fetching them into all eight.
+    int lo = 0, hi = n_queries, first = blockIdx.x, stride = gridDim.x;
+    if ((int)gridDim.x >= kXcds) {
+        const int x = blockIdx.x % kXcds;
+        lo = (int)((long long)n_queries * x / kXcds);
+        hi = (int)((long long)n_queries * (x + 1) / kXcds);
+        first = blockIdx.x / kXcds;
+        stride = ((int)gridDim.x - x + kXcds - 1) / kXcds;   // workgroups on this XCD
+    }
+    for (int k = lo + first; k < hi; k += stride) {
         const float *tree = trees + (size_t)(k / qpt) * D * cap;
         float q[D];
 #pragma unroll
