@@ -139,10 +139,21 @@ __device__ __forceinline__ float4 ld_obs(const float4 *p) {
     }
 }
 
-template <int N, bool EARLY, bool LOCAL = false>
+// GROUP: the N spheres all lie inside the ball (gx, gy, gz; gr) on every
+// lane -- the leaves of one link, bounded about the link's gate centre with a
+// 1 mm pad by the generator.  An obstacle whose own bounding ball (sphere:
+// itself; cuboid, capsule: the cull balls below) stays outside that ball on
+// every lane is at least 1 mm from every one of the N spheres, so all N of
+// its tests are positive and the wave skips the obstacle with one test.
+template <int N, bool EARLY, bool LOCAL = false, bool GROUP = false>
 __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
-                                          const float (&z)[N], const float (&r)[N], bool (&hit)[N]) {
+                                          const float (&z)[N], const float (&r)[N], bool (&hit)[N],
+                                          float gx = 0.0f, float gy = 0.0f, float gz = 0.0f, float gr = 0.0f) {
 #pragma clang fp contract(off)
+    auto group_far = [&](float cx, float cy, float cz, float cr) {
+        const float rb = gr + cr;
+        return ballot(!(sq3(cx - gx, cy - gy, cz - gz) > rb * rb)) == 0;
+    };
     // Each sphere keeps the smallest test value over the obstacles seen: some
     // value < 0 (a collision) iff the minimum is < 0.  One v_min per test
     // replaces a compare, a select and the bit packing of a boolean OR; a
@@ -162,6 +173,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
     };
     for (int i = 0; i < o.n_sph; i++) {                       // utils.cuh:58-81
         const float4 s = ld_obs<LOCAL>(o.sph + i);
+        if (GROUP && group_far(s.x, s.y, s.z, s.w)) continue;
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float rs = s.w + r[n];
@@ -178,6 +190,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         const float4 p = ld_obs<LOCAL>(o.cap + 2 * i), v = ld_obs<LOCAL>(o.cap + 2 * i + 1);
         const float mx = fmaf(0.5f, v.x, p.x), my = fmaf(0.5f, v.y, p.y), mz = fmaf(0.5f, v.z, p.z);
         const float cull = fmaf(0.5f * sqrtf(sq3(v.x, v.y, v.z)) + p.w, 1.0f + 0x1p-16f, 1e-3f);
+        if (GROUP && group_far(mx, my, mz, cull)) continue;
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float rb = r[n] + cull;
@@ -212,6 +225,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
     for (int i = 0; i < o.n_cub; i++) {                       // utils.cuh:158-175
         const float4 c = ld_obs<LOCAL>(o.cub + 4 * i), a1 = ld_obs<LOCAL>(o.cub + 4 * i + 1),
                      a2 = ld_obs<LOCAL>(o.cub + 4 * i + 2), a3 = ld_obs<LOCAL>(o.cub + 4 * i + 3);
+        if (GROUP && group_far(c.x, c.y, c.z, a3.w)) continue;
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
@@ -229,6 +243,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
         const float4 c = ld_obs<LOCAL>(o.zcub + 4 * i), a1 = ld_obs<LOCAL>(o.zcub + 4 * i + 1),
                      a2 = ld_obs<LOCAL>(o.zcub + 4 * i + 2);
         const float cull = ld_obs<LOCAL>(o.zcub + 4 * i + 3).w;
+        if (GROUP && group_far(c.x, c.y, c.z, cull)) continue;
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float xs = x[n] - c.x, ys = y[n] - c.y, zs = z[n] - c.z;
@@ -277,6 +292,17 @@ struct CheckCtx {
     template <int N, bool EARLY>
     __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
                                               const float (&r)[N], bool (&hit)[N]) const {
+        prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
+    }
+    // The N spheres lie inside the ball (gx, gy, gz; gr) on every lane.  The
+    // one-wave-per-edge kernels ignore the bound: it would hold the link
+    // frames live across the leaf loops (116 -> 256+ VGPRs for Panda, one
+    // wave per SIMD instead of four), which costs more than the skipped
+    // tests save; the planner's tasks use it (TaskCtx).
+    template <int N, bool EARLY>
+    __device__ __forceinline__ void env_multi_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                                      const float (&r)[N], bool (&hit)[N], float, float, float,
+                                                      float) const {
         prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
     }
     __device__ __forceinline__ bool self(float ax, float ay, float az, float bx, float by, float bz, float rs) const {

@@ -89,6 +89,12 @@ struct TaskCtx : CheckCtx {
                                               const float (&r)[N], bool (&hit)[N]) const {
         prrtc::env_multi<N, EARLY, true>(obs, x, y, z, r, hit);
     }
+    template <int N, bool EARLY>
+    __device__ __forceinline__ void env_multi_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                                      const float (&r)[N], bool (&hit)[N], float gx, float gy,
+                                                      float gz, float gr) const {
+        prrtc::env_multi<N, EARLY, true, true>(obs, x, y, z, r, hit, gx, gy, gz, gr);
+    }
     // CheckCtx::stop in path mode (a planner task is never in pose mode),
     // with the stop key shared across waves.
     __device__ __forceinline__ bool stop() {

@@ -98,3 +98,39 @@ def test_self_groups_dealt_once():
             masks = [sum(1 << i for i in range(12) if i % W == w) for w in range(W)]
             owners = [[w for w in range(W) if (masks[w] >> (j % 12)) & 1] for j in js]
             assert all(len(o) == 1 and o[0] == j % W for o, j in zip(owners, js))
+
+
+def test_group_bounds_hold_every_leaf(oracle_robots):
+    """Every bounded leaf group of the generated check sequences: on random
+    poses (oracle FK), each leaf sphere lies inside the emitted ball about
+    the gate centre with most of the 1 mm pad to spare, so skipping an
+    obstacle that stays outside the ball (plus its own bound) cannot change
+    a leaf test."""
+    from prrtc_amd.robots import limits, load_model
+    here = os.path.join(os.path.dirname(__file__), "..", "prrtc_amd", "csrc", "robots")
+    rng = np.random.default_rng(9)
+    for robot in ("panda", "fetch", "baxter"):
+        m = load_model(robot)
+        by_label = {c["label"]: c for c in m["env_checks"]}
+        text = open(os.path.join(here, robot + ".gen.h")).read()
+        blocks = re.findall(r"/\* env leaves: (.*?) \*/(.*?)(?=/\* env leaves: |/\* self: |\Z)", text, re.S)
+        lo, span = limits(robot)
+        q = (lo + span * rng.random((2000, len(lo)), dtype=np.float32)).astype(np.float32)
+        c = np.asarray(oracle_robots[robot].fk(q))
+        r = np.array([s["r"] for s in m["spheres"]])
+        bounded = set()
+        for label, body in blocks:
+            mb = re.search(r"env_multi_bounded<\d+, true>\(.*, (-?[0-9.e+-]+)f\);", body)
+            if not mb:
+                continue
+            bounded.add(label)
+            gr = float(mb.group(1))
+            chk = by_label[label]
+            g = chk["gate"]
+            reach = max(float((np.linalg.norm(c[:, l] - c[:, g], axis=1) + r[l]).max()) for l in chk["leaves"])
+            assert reach <= gr - 0.9e-3, (robot, label, reach, gr)
+        # exactly the checks of at least 6 leaves that all ride on the gate's frame
+        sph = m["spheres"]
+        want = {c["label"] for c in m["env_checks"] if not c["direct"] and len(c["leaves"]) >= 6 and
+                all(sph[l]["frame"] == sph[c["gate"]]["frame"] for l in c["leaves"])}
+        assert bounded == want, (robot, bounded ^ want)
