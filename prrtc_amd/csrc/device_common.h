@@ -295,10 +295,10 @@ struct CheckCtx {
         prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
     }
     // The N spheres lie inside the ball (gx, gy, gz; gr) on every lane.  The
-    // one-wave-per-edge kernels ignore the bound: it would hold the link
-    // frames live across the leaf loops (116 -> 256+ VGPRs for Panda, one
-    // wave per SIMD instead of four), which costs more than the skipped
-    // tests save; the planner's tasks use it (TaskCtx).
+    // one-wave-per-edge kernels ignore the bound: with it the Panda kernel
+    // needs 256 VGPRs plus AGPRs instead of 221 (one wave per SIMD instead
+    // of two), which costs more than the skipped tests save; the planner's
+    // tasks use it (TaskCtx).
     template <int N, bool EARLY>
     __device__ __forceinline__ void env_multi_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
                                                       const float (&r)[N], bool (&hit)[N], float, float, float,
