@@ -25,6 +25,8 @@ extern "C" {
 #define PRRTC_EOVERFLOW -4   /* a caller buffer (path, result) is too small */
 #define PRRTC_EBUSY -5       /* server queue full */
 #define PRRTC_EAGAIN -6      /* server result not ready yet */
+#define PRRTC_ETIMEDOUT -7   /* server request passed its deadline (result status) */
+#define PRRTC_ECANCELED -8   /* server request cancelled (result status) */
 
 /* Robot ids: the reference's explicit instantiations (src/planning/pRRTC.cu:768-771). */
 #define PRRTC_ROBOT_SPHERE 0 /* 3-DoF point robot, radius 1 (src/planning/Robots.hh:190-237) */
@@ -222,6 +224,11 @@ typedef struct prrtc_server_config {
     /* a request's slots are claimed ahead of older requests' for its first
      * young_rounds planner rounds (0: 64; < 0: never, plain round-robin) */
     int32_t young_rounds;
+    /* a request still unsolved this long after its submit ends unsolved
+     * with result status PRRTC_ETIMEDOUT (0: no deadline).  A queued request
+     * is dropped without running; a resident one stops at the end of its
+     * current planner round (its remaining tasks of that round are skipped) */
+    int32_t deadline_ms;
 } prrtc_server_config;
 
 typedef struct prrtc_server_stats {
@@ -235,6 +242,8 @@ typedef struct prrtc_server_stats {
     int64_t slices;           /* persistent launches */
     int64_t kernel_ns;        /* device time over all slices */
     int64_t edge_checks;
+    int64_t timed_out;        /* requests ended by the deadline */
+    int64_t cancelled;        /* requests ended by prrtc_server_cancel */
 } prrtc_server_stats;
 
 typedef struct prrtc_request_timing {
@@ -255,6 +264,11 @@ int prrtc_server_submit(prrtc_server *srv, const prrtc_problem *problem, const p
 int prrtc_server_poll(prrtc_server *srv, uint64_t ticket, int32_t wait_ms, prrtc_result *result, float *path,
                       int32_t path_cap, prrtc_request_timing *timing);
 int prrtc_server_get_stats(prrtc_server *srv, prrtc_server_stats *out);
+/* Ends a request unsolved with result status PRRTC_ECANCELED: at once if it
+ * is still queued, else at the end of its current planner round.  A request
+ * that has already finished keeps its result (PRRTC_OK, nothing changes);
+ * EINVAL for a ticket the server does not hold.  Poll the ticket as usual. */
+int prrtc_server_cancel(prrtc_server *srv, uint64_t ticket);
 /* Stops the worker after its current slice and frees the server; queued,
  * resident and unpolled requests are dropped. */
 int prrtc_server_destroy(prrtc_server *srv);

@@ -19,6 +19,10 @@
 
 #include "shapes.hh"
 
+// The reference header opens ppln at file scope (factory.hh:12); harnesses
+// written against it name robots::Panda and the like unqualified.
+using namespace ppln;
+
 namespace ppln::collision::factory {
 
 using ConstArrayRef = const std::array<float, 3> &;

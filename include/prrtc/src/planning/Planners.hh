@@ -126,11 +126,23 @@ PlannerResult<Robot> solve(typename Robot::Configuration &start, std::vector<typ
     const prrtc_settings cs = prrtc_detail::c_settings(settings);
     std::vector<float> goal_rows;
     for (auto &g : goals) goal_rows.insert(goal_rows.end(), g.begin(), g.end());
-    int cap = (int)std::min<long long>(2LL * settings.max_samples + 2, 1 << 20);
-    std::vector<float> path((std::size_t)cap * D);
+    // Paths are a few dozen rows: solve into a per-thread buffer of 1024
+    // rows, grown on demand.  A path that does not fit ends the solve with
+    // PRRTC_EOVERFLOW; the planner is deterministic in (problem, settings,
+    // seed), so solving again with room for the longest possible path (both
+    // trees full, pRRTC.cu:552-582) returns the same plan.
+    static thread_local std::vector<float> path;
+    const int full = (int)std::min<long long>(2LL * settings.max_samples + 2, 1 << 20);
+    int cap = std::min(full, 1024);
     prrtc_result r{};
-    const int rc = prrtc_solve(Robot::id, start.data(), goal_rows.data(), (int)goals.size(), &t.scene, &cs, &r,
-                               path.data(), cap);
+    int rc = PRRTC_OK;
+    while (true) {
+        if (path.size() < (std::size_t)cap * D) path.resize((std::size_t)cap * D);
+        rc = prrtc_solve(Robot::id, start.data(), goal_rows.data(), (int)goals.size(), &t.scene, &cs, &r,
+                         path.data(), cap);
+        if (rc != PRRTC_OK || r.status != PRRTC_EOVERFLOW || cap == full) break;
+        cap = full;
+    }
     if (rc != PRRTC_OK) {
         std::fprintf(stderr, "pRRTC::solve: %s\n", prrtc_last_error());
         return res;

@@ -400,22 +400,49 @@ typedef struct {
     int connected;
 } slot_out;
 
+/* Planted mistakes (tests only): or_set_plant(k) makes the planner commit
+ * mistake k, so the known-answer tests can show that each one is caught.
+ * 0 (the default) is the restatement itself. */
+enum {
+    PLANT_NONE = 0,
+    PLANT_ITER1_TREE,     /* first iteration extends the other tree */
+    PLANT_B1_RATIO,       /* balance 1: ratio test inverted */
+    PLANT_B1_SPLIT,       /* balance 1: near-equal sizes split the other way */
+    PLANT_B2_RULE,        /* balance 2: swap when the ratio is NOT below tree_ratio */
+    PLANT_DD_SIGN,        /* dynamic domain: grow and shrink factors exchanged */
+    PLANT_DD_SKIP,        /* dynamic domain: skip when the radius is above the distance */
+    PLANT_CHAIN_DROP,     /* connect: a partial chain keeps no nodes */
+    PLANT_STEER,          /* steering by min(1, d / range) instead of range / d */
+    PLANT_PATH_ORDER,     /* path returned start -> goal */
+    PLANT_N_EXT,          /* floor instead of ceil for the connect step count */
+    PLANT_COUNT
+};
+static int g_plant = PLANT_NONE;
+int or_set_plant(int k) {
+    if (k < 0 || k >= PLANT_COUNT) return -1;
+    g_plant = k;
+    return PLANT_COUNT;
+}
+
 static int choose_tree(const or_settings *st, int slot, int iter, int *t_prev, int n0, int n1) {
     int nnc = st->num_new_configs;
     int t = *t_prev;                                         /* pRRTC.cu:348-371 */
     if (st->balance == 0 || iter == 1) {
         t = (slot < (nnc / 2)) ? 0 : 1;
+        if (g_plant == PLANT_ITER1_TREE && iter == 1) t = 1 - t;
     } else if (st->balance == 1 && abs(n0 - n1) < 1.5 * nnc) {
         float ratio = n0 / (float)(n0 + n1);
         float bf = 1 - ratio;
         t = (slot < (nnc * bf)) ? 0 : 1;
+        if (g_plant == PLANT_B1_SPLIT) t = 1 - t;
     } else if (st->balance == 1) {
         float ratio = n0 / (float)(n0 + n1);
         t = (ratio < st->tree_ratio) ? 0 : 1;
+        if (g_plant == PLANT_B1_RATIO) t = 1 - t;
     } else if (st->balance == 2) {
         int nt = t == 0 ? n0 : n1, no = t == 0 ? n1 : n0;
         float ratio = abs(nt - no) / (float)nt;
-        if (ratio < st->tree_ratio) t = 1 - t;
+        if ((ratio < st->tree_ratio) != (g_plant == PLANT_B2_RULE)) t = 1 - t;
     }
     *t_prev = t;
     return t;
@@ -435,9 +462,12 @@ static void slot_step(const or_model *m, const or_scene *s, const or_settings *s
     float bsq;
     int nn = nearest(T, t, sizes[t], sample, &bsq);              /* pRRTC.cu:384-412 */
     float d = sqrtf(bsq);
-    float scale = fminf(1.0f, st->range / d);                   /* :415-418 */
+    float scale = fminf(1.0f, g_plant == PLANT_STEER ? d / st->range : st->range / d);   /* :415-418 */
     out->nn = nn;
-    if (st->dynamic_domain && T->radii[t][nn] < d) { out->skip = 1; return; }   /* :420-427 */
+    if (st->dynamic_domain && ((T->radii[t][nn] < d) != (g_plant == PLANT_DD_SKIP))) {    /* :420-427 */
+        out->skip = 1;
+        return;
+    }
     const float *nnv = T->nodes[t] + (size_t)nn * D;
     float delta[OR_MAX_DOF];
     for (int i = 0; i < D; i++) {                                 /* :430-434 */
@@ -457,7 +487,7 @@ static void slot_step(const or_model *m, const or_scene *s, const or_settings *s
 
     int nn_o = nearest(T, o, sizes[o], out->cfg, &bsq);          /* :478-505 */
     float d_o = sqrtf(bsq);
-    unsigned n_ext = (unsigned)ceilf(d_o / st->range);           /* :511 */
+    unsigned n_ext = (unsigned)(g_plant == PLANT_N_EXT ? floorf(d_o / st->range) : ceilf(d_o / st->range)); /* :511 */
     out->nn_o = nn_o;
     out->n_ext = n_ext;
     const float *nno = T->nodes[o] + (size_t)nn_o * D;
@@ -485,14 +515,17 @@ static void slot_step(const or_model *m, const or_scene *s, const or_settings *s
     }
     out->n_ok = (int)i_ext;
     out->connected = (i_ext == n_ext);
+    if (g_plant == PLANT_CHAIN_DROP && !out->connected) out->n_ok = 0;
 }
 
 static void dd_update(const or_settings *st, float *r, int grow) {
     if (!st->dynamic_domain) return;
+    const float up = g_plant == PLANT_DD_SIGN ? 1.f - st->dd_alpha : 1 + st->dd_alpha;
+    const float down = g_plant == PLANT_DD_SIGN ? 1 + st->dd_alpha : 1.f - st->dd_alpha;
     if (grow) {                                                   /* :456-468 */
-        if (*r != FLT_MAX) *r = *r * (1 + st->dd_alpha);
+        if (*r != FLT_MAX) *r = *r * up;
     } else {                                                      /* :584-598 */
-        *r = (*r == FLT_MAX) ? st->dd_radius : fmaxf(*r * (1.f - st->dd_alpha), st->dd_min_radius);
+        *r = (*r == FLT_MAX) ? st->dd_radius : fmaxf(*r * down, st->dd_min_radius);
     }
 }
 
@@ -553,6 +586,13 @@ static int commit(const or_model *m, const or_settings *st, trees_t *T, slot_out
     for (int i = n1 - 1; i >= 0; i--) memcpy(path_out + (size_t)row++ * D, p1 + (size_t)i * D, sizeof(float) * D);
     for (int i = 0; i < n0; i++) memcpy(path_out + (size_t)row++ * D, p0 + (size_t)i * D, sizeof(float) * D);
     memcpy(path_out + (size_t)row++ * D, start, sizeof(float) * D);
+    if (g_plant == PLANT_PATH_ORDER)
+        for (int i = 0; i < row / 2; i++)
+            for (int k = 0; k < D; k++) {
+                const float tmp = path_out[(size_t)i * D + k];
+                path_out[(size_t)i * D + k] = path_out[(size_t)(row - 1 - i) * D + k];
+                path_out[(size_t)(row - 1 - i) * D + k] = tmp;
+            }
     res->path_nodes = row;
     res->path_length = tn + on;
     res->cost = cost;

@@ -112,6 +112,10 @@ typedef struct {
 int or_plan(const or_model *m, const or_scene *s, const float *start, const float *goals, int n_goals,
             const or_settings *st, int mode, float *path_out, int path_cap, or_result *res);
 
+/* Tests only: make or_plan commit planted mistake k (0 = none); returns the
+ * number of mistakes defined, or -1 for an unknown k. */
+int or_set_plant(int k);
+
 /* Textbook RRT-Connect (Kuffner & LaValle 2000) with the same range and
  * per-edge granularity; uniform samples from SplitMix64(seed).  CPU baseline. */
 int or_rrt_connect(const or_model *m, const or_scene *s, const float *start, const float *goal,

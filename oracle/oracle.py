@@ -93,6 +93,8 @@ def lib():
         L.or_halton_init.argtypes = [vp, C.c_int, C.c_uint64, C.c_int]
         L.or_halton_next.argtypes = [vp, C.c_int, vp]
         L.or_set_fused.argtypes = [C.c_int]
+        L.or_set_plant.argtypes = [C.c_int]
+        L.or_set_plant.restype = C.c_int
         _lib = L
     return _lib
 
@@ -255,6 +257,18 @@ def make_settings(d: dict) -> Settings:
 def set_fused(on: bool) -> None:
     """Sphere tests rounded like the HIP kernels (True) or the reference (False)."""
     lib().or_set_fused(1 if on else 0)
+
+
+# Planted planner mistakes, in the order of the oracle's PLANT_* list (tests
+# show that the known answers catch each one).
+PLANTS = ("iter1_tree", "b1_ratio", "b1_split", "b2_rule", "dd_sign", "dd_skip", "chain_drop", "steer",
+          "path_order", "n_ext")
+
+
+def set_plant(name: str | None) -> None:
+    """Make or_plan commit a planted mistake (tests only); None restores it."""
+    k = 0 if name is None else PLANTS.index(name) + 1
+    assert lib().or_set_plant(k) == len(PLANTS) + 1
 
 
 def halton(dof: int, seed: int, slot: int, n: int) -> np.ndarray:

@@ -449,6 +449,7 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
         st->size[0] = sz[0];
         st->size[1] = sz[1];
         if (status == kStatusRunning && st->round >= d.max_iters) status = kStatusIterLimit;
+        if (status == kStatusRunning && stop_requested(ctrl, p)) status = kStatusStopped;
         st->status = status;
     }
     status = __builtin_amdgcn_readfirstlane(status);
@@ -628,12 +629,16 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 }
             }
             const int prio = p_out >= 0 ? task_priority(prio_policy, states[p_out].round, hot != 0) : 0;
+            // a problem the host has stopped: its claimed tasks are skipped
+            // (the round still completes and its commit ends the problem)
+            const int halt = p_out >= 0 && stop_requested(ctrl, p_out) ? 1 : 0;
             if (lane_id() == 0) {
                 ctl[0] = p_out;
                 ctl[1] = s_out;
                 ctl[2] = n_out;
                 ctl[3] = prio;
                 ctl[4] = crowd;
+                ctl[5] = halt;
             }
         }
         __syncthreads();
@@ -659,7 +664,17 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
         // One call site of plan_task and of plan_commit either way.
         const int w = wave_id();
         const bool runs = !crowd || w < n_slots;
-        if (runs) {
+        if (runs && ctl[5]) {
+            // stopped by the host: the slot's outcome is a skip (no nodes, no
+            // dynamic-domain update); one thread per slot writes it
+            if (lane_id() == 0 && (crowd || w == 0)) {
+                int *meta = d.meta + (size_t)(crowd ? s + w : s) * 8;
+                meta[0] = 0;
+                meta[1] = kSkip;
+                meta[2] = meta[3] = meta[4] = 0;
+            }
+            __threadfence();
+        } else if (runs) {
             ctx.obs = crowd ? scene_obs : scene_obs.slice(w, W);
             ctx.waves = crowd ? 1 : W;
             float *a = areas + (crowd ? (size_t)w * area : 0);

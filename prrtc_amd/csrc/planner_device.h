@@ -26,6 +26,7 @@ constexpr int kStatusSolved = 2;
 constexpr int kStatusIterLimit = -1;     // max_iters rounds without a solution
 constexpr int kStatusNodeLimit = -2;     // a tree would exceed max_samples
 constexpr int kStatusPathLimit = -3;     // solution path longer than the buffer
+constexpr int kStatusStopped = -4;       // stopped by the host (deadline or cancel)
 constexpr int kClaimedOut = 1 << 30;     // next_slot of a finished problem
 constexpr int kMaxExt = 255;             // longest connect chain (sub-edges)
 constexpr int kMaxNnc = 1024;
@@ -93,7 +94,16 @@ struct PlanControl {
     // slots are claimed first and its tasks run at raised wave priority, so
     // the longest problem of a batch is not slowed by the crowd
     unsigned long long hot_key;
+    // host-set stop word per problem (null: none): once nonzero, the
+    // problem's tasks are skipped as they are claimed and its next commit
+    // ends it with kStatusStopped (a solution found in that round still wins)
+    const int *stop_words;
 };
+
+__device__ __forceinline__ bool stop_requested(const PlanControl *ctrl, int p) {
+    return ctrl->stop_words &&
+           __hip_atomic_load(ctrl->stop_words + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
 
 // slot outcome flags in meta[1]
 constexpr int kSkip = 1, kExtOk = 2, kConnected = 4;

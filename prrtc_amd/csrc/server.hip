@@ -89,6 +89,9 @@ struct Request {
     size_t off = 0;
     int slices = 0;
     int round = 0;   // planner rounds done, as of the last harvest
+    int64_t deadline_ns = 0;   // absolute (now_ns clock), 0: none
+    bool cancel = false;       // prrtc_server_cancel asked to end it
+    int stop_reason = 0;       // PRRTC_ETIMEDOUT / PRRTC_ECANCELED once its stop word is set
     HostProblem hp() const { return HostProblem{robot, start.data(), goals.data(), n_goals, &scene, &st, 0}; }
 };
 
@@ -121,6 +124,13 @@ struct RobotClass {
     int mask_words = 0;
     float *d_paths = nullptr;
     int path_rows = 0;
+    // stop word per table entry (PlanControl::stop_words): set and cleared
+    // by the worker thread between slices, in pinned memory, and copied to
+    // the device before the next launch when one changed (the planner reads
+    // the device copy on every claim and commit)
+    int *h_stop = nullptr;
+    int *d_stop = nullptr;
+    bool stop_dirty = false;
     char *h_stage = nullptr;
     size_t stage_bytes = 0;
     std::vector<Request *> entries;   // resident request per table entry
@@ -143,7 +153,8 @@ struct RobotClass {
         PRRTC_HIP(hipMalloc(&d_list, sizeof(int) * E));
         mask_words = (E + 63) / 64;
         PRRTC_HIP(hipMalloc(&d_mask, sizeof(unsigned long long) * 3 * mask_words));
-        path_rows = E * std::max(1, path_cap - 2);
+        // prrtc_server_create bounds this product by INT32_MAX rows
+        path_rows = (int)((int64_t)E * std::max(1, path_cap - 2));
         PRRTC_HIP(hipMalloc(&d_paths, sizeof(float) * (size_t)path_rows * D));
         // status 0 marks a free entry the planner skips
         PRRTC_HIP(hipMemsetAsync(d_state, 0, sizeof(PlanState) * E, stream));
@@ -153,6 +164,10 @@ struct RobotClass {
         h_ctrl_down = pinned<PlanControl>(1);
         h_list = pinned<int>(E);
         h_mask = pinned<unsigned long long>(3 * mask_words);
+        h_stop = pinned<int>(E);
+        std::memset(h_stop, 0, sizeof(int) * (size_t)E);
+        PRRTC_HIP(hipMalloc(&d_stop, sizeof(int) * E));
+        PRRTC_HIP(hipMemsetAsync(d_stop, 0, sizeof(int) * E, stream));
         entries.assign(E, nullptr);
         for (int e = 0; e < E; e++) free_entries.insert(e);
         PRRTC_HIP(hipStreamSynchronize(stream));
@@ -173,6 +188,8 @@ struct RobotClass {
         (void)hipHostFree(h_ctrl_down);
         (void)hipHostFree(h_list);
         (void)hipHostFree(h_mask);
+        (void)hipHostFree(h_stop);
+        (void)hipFree(d_stop);
         if (h_stage) (void)hipHostFree(h_stage);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -215,6 +232,7 @@ struct ServerImpl {
 
     void run();
     void admit_locked();
+    void expire_locked(std::vector<std::pair<uint64_t, Finished>> &out);
     void launch(RobotClass &c);
     void harvest(RobotClass &c, std::vector<std::pair<uint64_t, Finished>> &out, std::vector<Request *> &gone,
                  int64_t &kernel_ns);
@@ -238,6 +256,10 @@ void ServerImpl::admit_locked() {
         c.free_entries.erase(c.free_entries.begin());
         c.high = std::max(c.high, rq->entry + 1);
         c.entries[rq->entry] = rq;
+        if (c.h_stop[rq->entry]) {   // no kernel of any robot runs during admission
+            c.h_stop[rq->entry] = 0;
+            c.stop_dirty = true;
+        }
         c.fresh.push_back(rq);
         rq->t_admit = now_ns();
         resident[rq->ticket] = std::move(*it);
@@ -247,6 +269,40 @@ void ServerImpl::admit_locked() {
     stats.resident = (int32_t)resident.size();
     stats.resident_bytes = arena->used();
     stats.peak_resident_bytes = std::max<uint64_t>(stats.peak_resident_bytes, arena->used());
+}
+
+// Deadlines and cancellations (worker thread, under the lock, between
+// slices): a queued request ends at once, unsolved and never admitted; a
+// resident one gets its stop word set, and the planner ends it at its next
+// commit (harvest reports it with the stop reason).
+void ServerImpl::expire_locked(std::vector<std::pair<uint64_t, Finished>> &out) {
+    const int64_t now = now_ns();
+    auto reason = [&](const Request &rq) {
+        if (rq.cancel) return PRRTC_ECANCELED;
+        return rq.deadline_ns && now >= rq.deadline_ns ? PRRTC_ETIMEDOUT : 0;
+    };
+    for (auto it = queue.begin(); it != queue.end();) {
+        Request *rq = it->get();
+        const int why = reason(*rq);
+        if (!why) { ++it; continue; }
+        Finished f;
+        f.dof = robot_dof(rq->robot);
+        f.r.status = why;
+        f.r.wall_ns = now - rq->t_submit;
+        f.timing.queued_ns = now - rq->t_submit;
+        out.emplace_back(rq->ticket, std::move(f));
+        it = queue.erase(it);
+    }
+    for (auto &kv : resident) {
+        Request *rq = kv.second.get();
+        if (rq->stop_reason || rq->entry < 0) continue;
+        if (const int why = reason(*rq)) {
+            rq->stop_reason = why;
+            cls[rq->robot].h_stop[rq->entry] = 1;
+            cls[rq->robot].stop_dirty = true;
+        }
+    }
+    stats.pending = (int32_t)queue.size();
 }
 
 void ServerImpl::launch(RobotClass &c) {
@@ -308,11 +364,16 @@ void ServerImpl::launch(RobotClass &c) {
     hc.crowd_above = plan_crowd_above(c.robot, max_nnc);
     hc.run_mask = c.d_mask;
     hc.open_mask = c.d_mask + W;
+    hc.stop_words = c.d_stop;
     if (cfg.young_rounds > 0) {
         hc.aged_mask = c.d_mask + 2 * W;
         hc.young_rounds = cfg.young_rounds;
     }
     PRRTC_HIP(hipMemcpyAsync(c.d_mask, c.h_mask, sizeof(unsigned long long) * 3 * W, hipMemcpyHostToDevice, s));
+    if (c.stop_dirty) {
+        PRRTC_HIP(hipMemcpyAsync(c.d_stop, c.h_stop, sizeof(int) * (size_t)c.E, hipMemcpyHostToDevice, s));
+        c.stop_dirty = false;
+    }
     PRRTC_HIP(hipMemcpyAsync(c.d_ctrl, c.h_ctrl_up, sizeof(PlanControl), hipMemcpyHostToDevice, s));
     // throughput shape while many problems are resident, latency shape
     // (obstacles split over the task's waves) for the last few
@@ -368,7 +429,10 @@ void ServerImpl::harvest(RobotClass &c, std::vector<std::pair<uint64_t, Finished
         r.goal_tree_size = ps.size[1];
         r.edge_checks = (int64_t)ps.edge_checks;
         r.solved = ps.status == kStatusSolved;
-        r.status = r.solved ? PRRTC_OK : (ps.status == kStatusPathLimit ? PRRTC_EOVERFLOW : PRRTC_OK);
+        r.status = r.solved                          ? PRRTC_OK
+                   : ps.status == kStatusPathLimit ? PRRTC_EOVERFLOW
+                   : ps.status == kStatusStopped   ? rq->stop_reason
+                                                   : PRRTC_OK;
         r.iters = r.solved ? ps.iters : 0;
         r.cost = r.solved ? ps.cost : 0.0f;
         if (r.solved) {
@@ -406,6 +470,15 @@ void ServerImpl::run() {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || !queue.empty() || !resident.empty(); });
                 if (stop) break;
+                std::vector<std::pair<uint64_t, Finished>> dropped;
+                expire_locked(dropped);
+                for (auto &kv : dropped) {
+                    stats.completed++;
+                    stats.timed_out += kv.second.r.status == PRRTC_ETIMEDOUT;
+                    stats.cancelled += kv.second.r.status == PRRTC_ECANCELED;
+                    done[kv.first] = std::move(kv.second);
+                }
+                if (!dropped.empty()) cv_done.notify_all();
                 admit_locked();
             }
             for (RobotClass &c : cls)
@@ -425,6 +498,8 @@ void ServerImpl::run() {
             for (auto &kv : out) {
                 stats.completed++;
                 stats.solved += kv.second.r.solved;
+                stats.timed_out += kv.second.r.status == PRRTC_ETIMEDOUT;
+                stats.cancelled += kv.second.r.status == PRRTC_ECANCELED;
                 stats.edge_checks += kv.second.r.edge_checks;
                 done[kv.first] = std::move(kv.second);
                 resident.erase(kv.first);
@@ -465,6 +540,7 @@ void prrtc_server_config_default(prrtc_server_config *cfg) {
     cfg->slice_us = 1000;
     cfg->path_cap = 1024;
     cfg->young_rounds = kYoungRounds;
+    cfg->deadline_ms = 0;
 }
 
 int prrtc_server_create(const prrtc_server_config *cfg, prrtc_server **out) {
@@ -480,10 +556,19 @@ int prrtc_server_create(const prrtc_server_config *cfg, prrtc_server **out) {
             if (cfg->slice_us) c.slice_us = cfg->slice_us;
             if (cfg->path_cap) c.path_cap = cfg->path_cap;
             if (cfg->young_rounds) c.young_rounds = cfg->young_rounds;
+            c.deadline_ms = cfg->deadline_ms;
         }
         if (c.max_resident < 1 || c.max_resident > (1 << 20) || c.max_pending < 1 || c.slice_us < 1 ||
-            c.path_cap < 2 || c.budget_bytes < 4096) {
+            c.path_cap < 2 || c.budget_bytes < 4096 || c.deadline_ms < 0) {
             set_error("prrtc_server_create: invalid config");
+            return PRRTC_EINVAL;
+        }
+        // each robot class keeps path_cap - 2 rows per resident entry; the
+        // pool is indexed with 32-bit row numbers on the device
+        const int64_t pool_rows = (int64_t)c.max_resident * std::max(1, c.path_cap - 2);
+        if (pool_rows > INT32_MAX) {
+            set_error("prrtc_server_create: max_resident x (path_cap - 2) = %lld path rows exceeds %d",
+                      (long long)pool_rows, INT32_MAX);
             return PRRTC_EINVAL;
         }
         auto srv = std::make_unique<prrtc_server>();
@@ -545,6 +630,7 @@ int prrtc_server_submit(prrtc_server *srv, const prrtc_problem *problem, const p
         if ((int)S.queue.size() >= S.cfg.max_pending) { set_error("prrtc_server_submit: queue full"); return PRRTC_EBUSY; }
         rq->ticket = S.next_ticket++;
         rq->t_submit = now_ns();
+        if (S.cfg.deadline_ms > 0) rq->deadline_ns = rq->t_submit + (int64_t)S.cfg.deadline_ms * 1000000;
         *ticket = rq->ticket;
         S.live.insert(rq->ticket);
         S.queue.push_back(std::move(rq));
@@ -591,6 +677,26 @@ int prrtc_server_get_stats(prrtc_server *srv, prrtc_server_stats *out) {
     std::lock_guard<std::mutex> lk(srv->impl.mu);
     *out = srv->impl.stats;
     return PRRTC_OK;
+}
+
+int prrtc_server_cancel(prrtc_server *srv, uint64_t ticket) {
+    return guarded("prrtc_server_cancel", [&]() -> int {
+        if (!srv) { set_error("prrtc_server_cancel: NULL server"); return PRRTC_EINVAL; }
+        ServerImpl &S = srv->impl;
+        std::lock_guard<std::mutex> lk(S.mu);
+        if (!S.live.count(ticket)) {
+            set_error("prrtc_server_cancel: unknown ticket %llu", (unsigned long long)ticket);
+            return PRRTC_EINVAL;
+        }
+        if (S.done.count(ticket)) return PRRTC_OK;   // finished: the result stands
+        // the worker applies it at its next pass (it owns the stop words)
+        for (auto &q : S.queue)
+            if (q->ticket == ticket) q->cancel = true;
+        auto it = S.resident.find(ticket);
+        if (it != S.resident.end()) it->second->cancel = true;
+        S.cv_work.notify_one();
+        return PRRTC_OK;
+    });
 }
 
 int prrtc_server_destroy(prrtc_server *srv) {

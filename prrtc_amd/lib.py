@@ -20,6 +20,8 @@ PRRTC_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "EBUSY", -6: "EAGAIN"}
 PRRTC_EBUSY = -5
 PRRTC_EAGAIN = -6
+PRRTC_ETIMEDOUT = -7
+PRRTC_ECANCELED = -8
 MAX_DOF = 14
 
 _f32p = C.POINTER(C.c_float)
@@ -74,7 +76,7 @@ class CTuning(C.Structure):
 class CServerConfig(C.Structure):
     _fields_ = [
         ("budget_bytes", C.c_uint64), ("max_resident", C.c_int32), ("max_pending", C.c_int32),
-        ("slice_us", C.c_int32), ("path_cap", C.c_int32), ("young_rounds", C.c_int32),
+        ("slice_us", C.c_int32), ("path_cap", C.c_int32), ("young_rounds", C.c_int32), ("deadline_ms", C.c_int32),
     ]
 
 
@@ -82,7 +84,8 @@ class CServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int64), ("completed", C.c_int64), ("solved", C.c_int64), ("pending", C.c_int32),
         ("resident", C.c_int32), ("resident_bytes", C.c_uint64), ("peak_resident_bytes", C.c_uint64),
-        ("slices", C.c_int64), ("kernel_ns", C.c_int64), ("edge_checks", C.c_int64),
+        ("slices", C.c_int64), ("kernel_ns", C.c_int64), ("edge_checks", C.c_int64), ("timed_out", C.c_int64),
+        ("cancelled", C.c_int64),
     ]
 
 
@@ -121,6 +124,7 @@ SIGNATURES = [
     ("prrtc_server_poll", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(CResult), C.c_void_p, C.c_int32,
                                     C.POINTER(CRequestTiming)]),
     ("prrtc_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(CServerStats)]),
+    ("prrtc_server_cancel", C.c_int, [C.c_void_p, C.c_uint64]),
     ("prrtc_server_destroy", C.c_int, [C.c_void_p]),
 ]
 
@@ -346,8 +350,9 @@ class Server:
     submit() copies the problem and returns a ticket; poll(ticket) returns
     (result, path, timing) once the problem has stopped, else None."""
 
-    def __init__(self, budget_bytes=0, max_resident=0, max_pending=0, slice_us=0, path_cap=0, young_rounds=0):
-        cfg = CServerConfig(budget_bytes, max_resident, max_pending, slice_us, path_cap, young_rounds)
+    def __init__(self, budget_bytes=0, max_resident=0, max_pending=0, slice_us=0, path_cap=0, young_rounds=0,
+                 deadline_ms=0):
+        cfg = CServerConfig(budget_bytes, max_resident, max_pending, slice_us, path_cap, young_rounds, deadline_ms)
         h = C.c_void_p()
         _check(load().prrtc_server_create(C.byref(cfg), C.byref(h)), "prrtc_server_create")
         self._h = h
@@ -376,6 +381,11 @@ class Server:
         del self._dof[ticket]
         timing = {name: getattr(tm, name) for name, _ in CRequestTiming._fields_}
         return _result_dict(res), path[:res.path_nodes].copy(), timing
+
+    def cancel(self, ticket: int) -> None:
+        """End the request unsolved (result status PRRTC_ECANCELED) unless it
+        has finished already; poll it as usual."""
+        _check(load().prrtc_server_cancel(self._h, ticket), "prrtc_server_cancel")
 
     def stats(self) -> dict:
         st = CServerStats()

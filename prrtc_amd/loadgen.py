@@ -37,7 +37,7 @@ def request_pool(n: int, seed: int = 0) -> list:
 
 def run_stream(pool: list, rate: float, seconds: float, *, arrival_seed: int = 11, budget_bytes: int = 16 << 30,
                slice_us: int = 1000, young_rounds: int = 0, settings: dict | None = None,
-               progress=None) -> dict:
+               deadline_ms: int = 0, progress=None) -> dict:
     """One Poisson stream of rate * seconds requests; returns its summary
     plus the per-request (request, result, path, timing) tuples under
     "_outs".  progress(done, total) is called about every 20 s while the
@@ -48,7 +48,8 @@ def run_stream(pool: list, rate: float, seconds: float, *, arrival_seed: int = 1
     n_req = max(1, int(rate * seconds))
     arrivals = np.cumsum(np.random.default_rng(arrival_seed).exponential(1.0 / rate, n_req))
     submitted, rejected = [], 0
-    with P.Server(budget_bytes=budget_bytes, slice_us=slice_us, young_rounds=young_rounds) as srv:
+    with P.Server(budget_bytes=budget_bytes, slice_us=slice_us, young_rounds=young_rounds,
+                  deadline_ms=deadline_ms) as srv:
         t_start = time.perf_counter()
         for i, a in enumerate(arrivals):
             delay = t_start + a - time.perf_counter()
@@ -89,12 +90,14 @@ def run_stream(pool: list, rate: float, seconds: float, *, arrival_seed: int = 1
         "p50_ms_solved": pct(lat_solved, 50), "p99_ms_solved": pct(lat_solved, 99),
         "admission_wait_p50_ms": pct(queued, 50), "admission_wait_p99_ms": pct(queued, 99),
         "drain_s": t_end - t_arrived, "wall_s": wall, "slices": sstats["slices"],
+        "deadline_ms": deadline_ms, "timed_out": sstats["timed_out"],
         "peak_resident_mib": sstats["peak_resident_bytes"] / 2**20,
         "per_robot": {k: {"requests": v["requests"], "solved": v["solved"], "p50_ms": pct(v["lat"], 50),
                           "p99_ms": pct(v["lat"], 99)} for k, v in by_robot.items()},
         "settings": "g {granularity}, nnc {num_new_configs}, range {range}, balance {balance}, max_iters {max_iters}; "
-                    "{slice} us slices, {budget} GiB budget, young_rounds {young}".format(
-                        **st, slice=slice_us, budget=budget_bytes >> 30, young=young_rounds or 64),
+                    "{slice} us slices, {budget} GiB budget, young_rounds {young}, deadline {dl}".format(
+                        **st, slice=slice_us, budget=budget_bytes >> 30, young=young_rounds or 64,
+                        dl=f"{deadline_ms} ms" if deadline_ms else "none"),
         "_outs": outs,
     }
 
@@ -105,6 +108,7 @@ def main(argv=None):
     ap.add_argument("--seconds", type=float, default=10.0, help="arrival window per rate")
     ap.add_argument("--pool", type=int, default=2000, help="distinct seeded problems")
     ap.add_argument("--young-rounds", type=int, default=0)
+    ap.add_argument("--deadline-ms", type=int, default=0, help="per-request deadline (0: none)")
     ap.add_argument("--out", default=None, help="write the summaries as JSON")
     args = ap.parse_args(argv)
     import prrtc_amd as P
@@ -114,11 +118,12 @@ def main(argv=None):
     rows = []
     for rate in (float(r) for r in args.rates.split(",")):
         print(f"[loadgen] {rate:.0f} req/s for {args.seconds:.0f} s", file=sys.stderr, flush=True)
-        r = run_stream(pool, rate, args.seconds, young_rounds=args.young_rounds,
+        r = run_stream(pool, rate, args.seconds, young_rounds=args.young_rounds, deadline_ms=args.deadline_ms,
                        progress=lambda d, n: print(f"[loadgen] {d}/{n} results", file=sys.stderr, flush=True))
         r.pop("_outs")
         print(f"[loadgen] {rate:.0f}/s: {r['solved']}/{r['requests']} solved ({r['rejected']} rejected), "
               f"{r['plans_per_s']:.1f} plans/s, p50 {r['p50_ms']:.1f} ms, p99 {r['p99_ms']:.1f} ms, "
+              f"{r['timed_out']} timed out, "
               f"drain {r['drain_s']:.1f} s", file=sys.stderr, flush=True)
         rows.append(r)
     if args.out:

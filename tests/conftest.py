@@ -26,6 +26,28 @@ def golden_scene(npz, prefix):
     return Scene.from_arrays({k: npz[f"{prefix}__{k}"] for k in SCENE_KEYS})
 
 
+NEAR = 1e-6   # poses closer than this to contact may flip verdicts (float rounding)
+
+
+def edge_poses(edges, g):
+    """Poses a + (k+1)(b-a)/g, k = 0..g-1, of each edge (pRRTC.cu:438-439), float32."""
+    a, b = edges[:, 0], edges[:, 1]
+    delta = (b - a) / np.float32(g)
+    k = np.arange(1, g + 1, dtype=np.float32)[None, :, None]
+    return (a[:, None, :] + k * delta[:, None, :]).astype(np.float32)
+
+
+def edge_margins(R, scene, edges, g):
+    """Smallest distance to contact (m) over the poses of each edge, from the
+    oracle's exhaustive clearance (every sphere against every obstacle and
+    self pair): an edge verdict may only flip where this is below NEAR."""
+    if len(edges) == 0:
+        return np.zeros(0, np.float32)
+    q = edge_poses(np.asarray(edges, np.float32), g)
+    _, m = R.check(scene, q.reshape(-1, q.shape[-1]), margin=True)
+    return m.reshape(len(edges), g).min(axis=1)
+
+
 @pytest.fixture(scope="session")
 def oracle_robots():
     from oracle import oracle

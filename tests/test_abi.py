@@ -98,6 +98,10 @@ def test_server_rejects_bad_config_without_gpu():
     assert L.prrtc_server_create(C.byref(cfg), C.byref(h)) == -1
     assert not h.value
     assert L.prrtc_server_create(None, None) == -1
+    # path pool of max_resident x (path_cap - 2) rows: 2^20 x 4094 > INT32_MAX
+    cfg = CServerConfig(0, 1 << 20, 0, 0, 4096, 0)
+    assert L.prrtc_server_create(C.byref(cfg), C.byref(h)) == -1 and not h.value
+    assert b"path rows" in L.prrtc_last_error()
 
 
 def test_cuboid_cull_radius():

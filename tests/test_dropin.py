@@ -27,6 +27,40 @@ def test_dropin_harness_compiles_and_links(tmp_path):
     compile_cpp("drop_in.cpp", str(tmp_path / "drop_in"))
 
 
+def test_unqualified_robot_names_compile(tmp_path):
+    """robots::Panda unqualified after `using namespace ppln::collision;`, as
+    the reference's scripts/single_mbm.cpp:142 and evaluate_mbm.cpp:183 write
+    it: the reference's factory.hh:12 opens ppln at file scope, so must ours."""
+    exe = compile_cpp("unqualified.cpp", str(tmp_path / "unqualified"))
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert out.split() == ["robots:", "7", "8", "14"]
+
+
+REF_SCRIPTS = "/root/reference/scripts"
+NLOHMANN = "/opt/conda/include/json.hpp"
+
+
+@pytest.mark.skipif(not (os.path.isdir(REF_SCRIPTS) and os.path.isfile(NLOHMANN)),
+                    reason="needs the reference's MBM scripts and an nlohmann json header")
+@pytest.mark.parametrize("script", ["single_mbm.cpp", "evaluate_mbm.cpp"])
+def test_reference_harness_compiles_unchanged(tmp_path, script):
+    """The reference's own MBM harnesses, syntax-checked against the drop-in
+    headers.  The reference fetches a newer nlohmann json than the one in
+    this image (3.1.1); the only error allowed is that library's lack of
+    structured bindings over items() (evaluate_mbm.cpp:126), never one from
+    the drop-in headers or an undeclared name."""
+    inc = tmp_path / "nl" / "nlohmann"
+    inc.mkdir(parents=True)
+    os.symlink(NLOHMANN, inc / "json.hpp")
+    res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + INC, "-I" + str(tmp_path / "nl"),
+                          os.path.join(REF_SCRIPTS, script)], capture_output=True, text=True)
+    errors = [l for l in res.stderr.splitlines() if ": error:" in l]
+    bad = [l for l in errors if "cannot decompose inaccessible member" not in l]
+    assert not bad, "\n".join(bad)
+    if script == "single_mbm.cpp":
+        assert res.returncode == 0, res.stderr
+
+
 def test_header_factory_matches_python(tmp_path):
     from prrtc_amd import scenes as S
     exe = compile_cpp("factory_dump.cpp", str(tmp_path / "factory_dump"), link=False)

@@ -102,3 +102,85 @@ def test_server_errors(gpu, oracle_robots):
         t = srv.submit(p, dict(st, max_samples=2000))    # fits
         r, path, tm = srv.poll(t)
         assert r["status"] == 0
+
+
+def _sealed_request():
+    """A point-robot problem that never connects: the start sits inside a
+    sphere, so every edge from the start tree collides and the goal tree can
+    never reach it (rounds run until max_iters)."""
+    from prrtc_amd import scenes as S
+    sc = S.Scene.build(spheres=[S.sphere((-3.5, 0, 0), 2.2)])
+    p = dict(robot="sphere", start=np.array([-3.5, 0, 0], np.float32), goals=np.array([[3.5, 0.5, 0]], np.float32),
+             scene=sc)
+    st = dict(num_new_configs=32, granularity=16, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0,
+              max_samples=20000, max_iters=1_000_000, seed=3)
+    return p, st
+
+
+def test_server_deadline(gpu, oracle_robots):
+    """A resident request that never connects ends at its deadline, unsolved
+    with status ETIMEDOUT (instead of running a million rounds); a request
+    queued behind a full budget past its deadline is dropped without
+    running; a solvable request beside them returns its solo result."""
+    import time
+    from prrtc_amd.lib import PRRTC_ETIMEDOUT
+    sealed, sst = _sealed_request()
+    (p, st), = mixed_requests(oracle_robots, 1, seed0=3)
+    alone = gpu.solve(p["robot"], p["start"], p["goals"], p["scene"], st)
+    with gpu.Server(slice_us=500, deadline_ms=60) as srv:
+        t0 = time.perf_counter()
+        ts = srv.submit(sealed, sst)
+        tp = srv.submit(p, st)
+        r, path, tm = srv.poll(ts)
+        waited = time.perf_counter() - t0
+        r1, path1, _ = srv.poll(tp)
+        stats = srv.stats()
+    print(f"sealed request: status {r['status']}, {tm['slices']} slices, {waited * 1e3:.1f} ms", flush=True)
+    assert not r["solved"] and r["status"] == PRRTC_ETIMEDOUT and r["start_tree_size"] == 1
+    assert 0.06 <= waited < 2.0
+    assert_same(r1, path1, *alone, "beside a timed-out request")
+    assert stats["timed_out"] == 1 and stats["completed"] == 2
+    # a budget that holds one sealed problem: the second waits in the queue
+    # until its deadline and is dropped unadmitted
+    one = int(1.5 * 2 * 20000 * 5 * 4)
+    with gpu.Server(slice_us=500, deadline_ms=40, budget_bytes=one) as srv:
+        ta, tb = srv.submit(sealed, sst), srv.submit(sealed, sst)
+        ra, _, tma = srv.poll(ta)
+        rb, _, tmb = srv.poll(tb)
+    assert ra["status"] == rb["status"] == PRRTC_ETIMEDOUT
+    assert tmb["slices"] == 0 and tmb["resident_ns"] == 0 and tmb["queued_ns"] >= 40e6
+
+
+def test_server_cancel(gpu, oracle_robots):
+    """prrtc_server_cancel ends a resident request (status ECANCELED) within
+    a few slices; cancelling a finished request keeps its result; an unknown
+    ticket is EINVAL."""
+    import time
+    from prrtc_amd.lib import PRRTC_ECANCELED
+    sealed, sst = _sealed_request()
+    (p, st), = mixed_requests(oracle_robots, 1, seed0=4)
+    alone = gpu.solve(p["robot"], p["start"], p["goals"], p["scene"], st)
+    with gpu.Server(slice_us=500) as srv:
+        ts = srv.submit(sealed, sst)
+        tp = srv.submit(p, st)
+        time.sleep(0.03)
+        assert srv.poll(ts, 0) is None                  # still running
+        t0 = time.perf_counter()
+        srv.cancel(ts)
+        r, _, tm = srv.poll(ts)
+        took = time.perf_counter() - t0
+        r1, path1, _ = srv.poll(tp)
+        stats = srv.stats()
+        with pytest.raises(gpu.PrrtcError, match="EINVAL"):
+            srv.cancel(987654)
+    print(f"cancelled after {tm['slices']} slices, {took * 1e3:.1f} ms after the call", flush=True)
+    assert not r["solved"] and r["status"] == PRRTC_ECANCELED and took < 1.0
+    assert_same(r1, path1, *alone, "beside a cancelled request")
+    assert stats["cancelled"] == 1
+    with gpu.Server(slice_us=500) as srv:               # cancel after the result: it stands
+        t = srv.submit(p, st)
+        while srv.stats()["completed"] < 1:
+            time.sleep(0.005)
+        srv.cancel(t)
+        r2, path2, _ = srv.poll(t)
+    assert_same(r2, path2, *alone, "cancelled after finishing")

@@ -5,11 +5,9 @@ import zlib
 import numpy as np
 import pytest
 
-from conftest import ROBOTS, golden, golden_scene
+from conftest import NEAR, ROBOTS, edge_margins, golden, golden_scene
 
 pytestmark = pytest.mark.gpu
-
-NEAR = 1e-6
 
 
 @pytest.mark.parametrize("robot", ROBOTS)
@@ -70,7 +68,12 @@ def test_edge_verdicts_oracle(robot, g, gpu, oracle_robots):
     want = R.validate_edges(sc, edges, g)
     got = gpu.validate_edges(robot, sc, edges, g)
     assert 0 < want.sum() < len(want) or len(want) < 20
-    assert (got != want).sum() <= 1, f"{(got != want).sum()} of {len(want)} edges differ"
+    # a verdict may differ only on an edge with a pose within NEAR of contact
+    flips = np.nonzero(got != want)[0]
+    m = edge_margins(R, sc, edges[flips], g)
+    print(f"{robot}/g{g}: {len(flips)} edge flips of {len(want)}, all within {NEAR} m of contact: "
+          f"{bool(np.all(m < NEAR))}")
+    assert np.all(m < NEAR), f"edges {flips[m >= NEAR].tolist()} differ {m[m >= NEAR]} m from contact"
 
 
 def test_empty_inputs(gpu):
