@@ -84,17 +84,23 @@ def main():
     ap.add_argument("--edges", type=int, default=1_000_000)
     ap.add_argument("--granularity", type=int, default=128)
     ap.add_argument("--plans", type=int, default=16, help="single-problem plans for p50/p99")
-    ap.add_argument("--c0-seeds", type=int, default=128)
+    ap.add_argument("--c0-seeds", type=int, default=128, help="C0-style problems (problem and planner seed i)")
+    ap.add_argument("--c0-rounds", type=int, default=20000, help="max_iters of each C0 problem")
+    ap.add_argument("--batch-rounds", type=int, default=3000,
+                    help="max_iters of every problem of the batched vs plain comparisons (both forms)")
     ap.add_argument("--batch", type=int, default=256, help="Fetch problems of the batched vs plain comparison")
     ap.add_argument("--robot-batch", type=int, default=96,
                     help="Panda and Baxter shelf problems of their batched vs plain comparisons")
     ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-deadline-ms", type=int, default=250, help="per-request deadline of the server (0: none)")
     ap.add_argument("--serve-young", type=int, default=0,
                     help="server young_rounds: rounds a request is served ahead of older ones (0: default 64, "
                          "-1: round-robin)")
     ap.add_argument("--cpu-edges", type=int, default=500_000)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (at most the CPUs this process may use)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plans", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
@@ -304,60 +310,77 @@ def main():
             f"p99 {out['plan']['p99_ms']:.1f} ms")
 
         # ------------------------------------------------------------- c0
-        c0 = workloads.c0_problem(0)
+        # C0-style problems, one per seed (a fresh 12-sphere scene, start and
+        # goal each): the seed changes the problem, so the figures are
+        # distributions over problems, not one plan repeated
         c0_st = dict(num_new_configs=128, granularity=32, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0,
-                     max_samples=100000, max_iters=1000000)
-        times, costs, solved = [], [], 0
-        for seed in range(args.c0_seeds):
+                     max_samples=100000, max_iters=args.c0_rounds)
+        c0_probs = [workloads.c0_problem(s) for s in range(args.c0_seeds)]
+        P.solve("panda", c0_probs[0]["start"], c0_probs[0]["goals"], c0_probs[0]["scene"], c0_st)   # warm
+        times, costs, rounds, c0_res = [], [], [], []
+        for seed, p in enumerate(c0_probs):
             t0 = time.perf_counter()
-            res, path = P.solve("panda", c0["start"], c0["goals"], c0["scene"], dict(c0_st, seed=seed))
+            res, path = P.solve("panda", p["start"], p["goals"], p["scene"], dict(c0_st, seed=seed))
             times.append((time.perf_counter() - t0) * 1e3)
-            solved += res["solved"]
+            c0_res.append(res)
             if res["solved"]:
                 costs.append(res["cost"])
-                paths_to_check.append(("panda", c0["scene"], path, 32))
-        out["c0"] = {"seeds": args.c0_seeds, "solved": solved, "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
-                     "cost_p50": pct(costs, 50), "cost_p90": pct(costs, 90),
-                     "settings": "panda, 12 spheres, nnc 128, g 32, range 0.5, balance 2, ratio 1, dd off"}
-        log(f"c0: {solved}/{args.c0_seeds} solved, p50 {out['c0']['p50_ms']:.2f} ms, p99 {out['c0']['p99_ms']:.2f} ms")
+                rounds.append(res["iters"])
+                paths_to_check.append(("panda", p["scene"], path, 32))
+        solved = len(costs)
+        out["c0"] = {"problems": len(c0_probs), "solved": solved, "success_rate": solved / len(c0_probs),
+                     "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
+                     "cost_p10": pct(costs, 10), "cost_p50": pct(costs, 50), "cost_p90": pct(costs, 90),
+                     "rounds_p50": pct(rounds, 50), "rounds_p90": pct(rounds, 90),
+                     "settings": f"panda, 12 spheres, problem and planner seed i for i < {len(c0_probs)}, nnc 128, "
+                                 f"g 32, range 0.5, balance 2, ratio 1, dd off, max_iters {args.c0_rounds}"}
+        log(f"c0: {solved}/{len(c0_probs)} solved, p50 {out['c0']['p50_ms']:.2f} ms, p99 {out['c0']['p99_ms']:.2f} ms, "
+            f"cost p10/p50/p90 {out['c0']['cost_p10']:.2f}/{out['c0']['cost_p50']:.2f}/{out['c0']['cost_p90']:.2f}")
 
         # -------------------------------------------------- batched vs plain
-        log(f"batched vs plain: {args.batch} Fetch problems")
+        # all 256 Fetch problems, both forms under the same round budget
+        # (max_iters): a problem that never connects costs its budget in
+        # both.  Each form runs the num_new_configs the autotuner picked for it.
+        log(f"batched vs plain: {args.batch} Fetch problems, {args.batch_rounds} rounds each at most")
         bprobs = workloads.c2_problems(args.batch, seed=0)
         base = dict(granularity=64, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0, max_samples=100000,
                     seed=1)
         nnc_b = tuned.get("batch_nnc", {}).get("fetch", 128)
         nnc_p = tuned.get("plain_nnc", {}).get("fetch", 256)
-        P.solve_batch(bprobs[:8], dict(base, num_new_configs=nnc_b, max_iters=50))            # warm
-        # problems the planner connects within 3000 rounds at nnc 64 (a fixed
-        # property of each problem: start and goal in one free-space component)
-        pres, _, _ = P.solve_batch(bprobs, dict(base, num_new_configs=64, max_iters=3000))
-        feas = [p for p, r in zip(bprobs, pres) if r["solved"]]
-        bst = dict(base, num_new_configs=nnc_b, max_iters=20000)
-        pst = dict(base, num_new_configs=nnc_p, max_iters=20000)
+        bst = dict(base, num_new_configs=nnc_b, max_iters=args.batch_rounds)
+        pst = dict(base, num_new_configs=nnc_p, max_iters=args.batch_rounds)
+        P.solve_batch(bprobs[:8], dict(bst, max_iters=50))            # warm
         t0 = time.perf_counter()
-        bres, bpaths, bstats = P.solve_batch(feas, bst)
+        bres, bpaths, bstats = P.solve_batch(bprobs, bst)
         t_batch = time.perf_counter() - t0
         t0 = time.perf_counter()
-        plain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], pst) for p in feas]
+        plain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], pst) for p in bprobs]
         t_plain = time.perf_counter() - t0
         # the non-persistent comparator: the same batch as graph-replayed round grids
         t0 = time.perf_counter()
-        gres, _, gstats = P.solve_batch(feas, bst, graph_rounds=True)
+        gres, _, gstats = P.solve_batch(bprobs, bst, graph_rounds=True)
         t_graph = time.perf_counter() - t0
         same_graph = all(r["iters"] == g["iters"] and r["cost"] == g["cost"] for r, g in zip(bres, gres))
-        sub = min(32, len(feas))
+        sub = min(32, len(bprobs))
         same = all(r["iters"] == pr[0]["iters"] and pa.tobytes() == pr[1].tobytes() for r, pa, pr in
                    zip(bres[:sub], bpaths[:sub], [P.solve(p["robot"], p["start"], p["goals"], p["scene"], bst)
-                                                  for p in feas[:sub]]))
+                                                  for p in bprobs[:sub]]))
         n_b = sum(r["solved"] for r in bres)
         n_p = sum(r[0]["solved"] for r in plain)
-        out["batched"] = {"problems": len(bprobs), "feasible": len(feas), "solved_batched": n_b, "solved_plain": n_p,
+        # k_plan: algorithmic work estimate of the batch's validation passes
+        # (poses put on lanes x (FK + every gate sphere against every
+        # obstacle) per problem) over the persistent kernel's time
+        plan_flops = sum(r["pose_checks"] * accounting.gate_pass_flops("fetch", p["scene"])
+                         for p, r in zip(bprobs, bres))
+        plan_s = bstats["kernel_ns"] / 1e9
+        out["batched"] = {"problems": len(bprobs), "round_budget": args.batch_rounds,
+                          "solved_batched": n_b, "solved_plain": n_p,
                           "plans_per_s_batched": n_b / t_batch, "plans_per_s_plain": n_p / t_plain,
                           "speedup": (n_b / t_batch) / max(1e-9, n_p / t_plain),
                           "batch_wall_s": t_batch, "plain_wall_s": t_plain,
                           "batch_kernel_ms": bstats["kernel_ns"] / 1e6, "rounds_max": bstats["rounds"],
-                          "edge_checks_per_s_batched": bstats["edge_checks"] / (bstats["kernel_ns"] / 1e9),
+                          "edge_checks_per_s_batched": bstats["edge_checks"] / plan_s,
+                          "pose_checks_per_s_batched": bstats["pose_checks"] / plan_s,
                           "plans_per_s_graph_rounds": sum(r["solved"] for r in gres) / t_graph,
                           "graph_rounds_wall_s": t_graph, "graph_rounds_kernel_ms": gstats["kernel_ns"] / 1e6,
                           "graph_rounds_identical": bool(same_graph),
@@ -365,46 +388,83 @@ def main():
                           "cost_p50": pct([r["cost"] for r in bres if r["solved"]], 50),
                           "cost_p90": pct([r["cost"] for r in bres if r["solved"]], 90),
                           "same_settings_identical_results": bool(same),
-                          "settings": "fetch, g 64, range 0.5, balance 2, dd off; nnc per mode from the autotuner"}
+                          "settings": f"fetch, g 64, range 0.5, balance 2, dd off, max_iters {args.batch_rounds} in "
+                                      "both forms; nnc per form from the autotuner"}
         log(f"batched {out['batched']['plans_per_s_batched']:.1f} plans/s vs plain "
-            f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x), identical={same}; "
-            f"graph-replayed rounds {out['batched']['plans_per_s_graph_rounds']:.1f} plans/s, "
-            f"identical={same_graph}")
+            f"{out['batched']['plans_per_s_plain']:.1f} plans/s ({out['batched']['speedup']:.2f}x; {n_b}/{n_p} of "
+            f"{len(bprobs)} solved), identical={same}; graph-replayed rounds "
+            f"{out['batched']['plans_per_s_graph_rounds']:.1f} plans/s, identical={same_graph}")
+        # the problems the batch connected, again without the round budget
+        # (max_iters 20000): the ratio of the solvable work alone
+        feas = [p for p, r in zip(bprobs, bres) if r["solved"]]
+        t0 = time.perf_counter()
+        fres, _, fstats = P.solve_batch(feas, dict(bst, max_iters=20000))
+        tf_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        fplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], dict(pst, max_iters=20000))
+                  for p in feas]
+        tf_plain = time.perf_counter() - t0
+        nf_b = sum(r["solved"] for r in fres)
+        nf_p = sum(r[0]["solved"] for r in fplain)
+        out["batched"]["connectable"] = {
+            "problems": len(feas), "solved_batched": nf_b, "solved_plain": nf_p,
+            "plans_per_s_batched": nf_b / tf_batch, "plans_per_s_plain": nf_p / tf_plain,
+            "speedup": (nf_b / tf_batch) / max(1e-9, nf_p / tf_plain), "batch_kernel_ms": fstats["kernel_ns"] / 1e6,
+            "hardest_alone_ms": max(r[0]["kernel_ns"] for r in fplain) / 1e6,
+            "settings": "the problems the budgeted batch solved, max_iters 20000 in both forms"}
+        log(f"connectable subset: {nf_b / tf_batch:.1f} vs {nf_p / tf_plain:.1f} plans/s "
+            f"({out['batched']['connectable']['speedup']:.2f}x) on {len(feas)} problems; hardest alone "
+            f"{out['batched']['connectable']['hardest_alone_ms']:.1f} ms, batch {fstats['kernel_ns'] / 1e6:.1f} ms")
+        out["k_plan_roofline"] = {
+            "kernel": "k_plan<FetchRobot>", "workload": f"the {len(bprobs)}-problem Fetch batch above",
+            "bound": "valu", "achieved": plan_flops / plan_s / 1e12, "peak": PEAK_FP32_PACKED / 1e12,
+            "unit": "TFLOP/s", "frac": plan_flops / plan_s / PEAK_FP32_PACKED,
+            "frac_of_nonpacked_peak": plan_flops / plan_s / PEAK_FP32, "kernel_ms": plan_s * 1e3,
+            "pose_checks": bstats["pose_checks"], "flops_per_pose_mean": plan_flops / max(1, bstats["pose_checks"]),
+            "flops_definition": "per pose put on a lane: FK + every env gate sphere against every obstacle of the "
+                                "problem's scene (leaves, self pairs and early exits not counted)",
+            "valu_counters": None}
+        kpath = os.path.join(ROOT, "profiles", "planner_fetch_c2_counters.json")
+        if os.path.exists(kpath):
+            with open(kpath) as f:
+                out["k_plan_roofline"]["valu_counters"] = json.load(f).get("summary")
+        log(f"k_plan: {bstats['pose_checks']:.3e} poses, {out['k_plan_roofline']['achieved']:.2f} TFLOP/s "
+            f"estimated ({out['k_plan_roofline']['frac'] * 100:.1f}% of packed peak)")
 
         # ------------------------------ batched vs plain, Panda and Baxter
         # The same comparison on the shelf problem sets of the uneven mix,
-        # one class at a time, with each class's settings and tuned nnc: both
-        # forms solve the same connectable problems at the same granularity.
+        # one class at a time, with each class's settings and tuned nnc,
+        # both forms under one round budget.
         from prrtc_amd.autotune import SETTINGS as CLASS_SETTINGS, problems as class_problems
         out["batched_per_robot"] = {}
         for robot in ("panda", "baxter"):
             rp = class_problems(robot, args.robot_batch)
             cbase = CLASS_SETTINGS[robot]
-            rpres, _, _ = P.solve_batch(rp, dict(cbase, num_new_configs=64, max_iters=2000))
-            rf = [p for p, r in zip(rp, rpres) if r["solved"]]
-            rb = dict(cbase, num_new_configs=tuned.get("batch_nnc", {}).get(robot, 128), max_iters=20000)
-            rpl = dict(cbase, num_new_configs=tuned.get("plain_nnc", {}).get(robot, 128), max_iters=20000)
+            rb = dict(cbase, num_new_configs=tuned.get("batch_nnc", {}).get(robot, 128), max_iters=args.batch_rounds)
+            rpl = dict(cbase, num_new_configs=tuned.get("plain_nnc", {}).get(robot, 128),
+                       max_iters=args.batch_rounds)
             t0 = time.perf_counter()
-            rres, rpaths, _ = P.solve_batch(rf, rb)
+            rres, rpaths, _ = P.solve_batch(rp, rb)
             tb = time.perf_counter() - t0
             t0 = time.perf_counter()
-            rplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], rpl) for p in rf]
+            rplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], rpl) for p in rp]
             tp = time.perf_counter() - t0
             nb_r = sum(r["solved"] for r in rres)
             np_r = sum(r[0]["solved"] for r in rplain)
             out["batched_per_robot"][robot] = {
-                "problems": len(rp), "feasible": len(rf), "solved_batched": nb_r, "solved_plain": np_r,
+                "problems": len(rp), "solved_batched": nb_r, "solved_plain": np_r,
                 "plans_per_s_batched": nb_r / tb, "plans_per_s_plain": np_r / tp,
                 "speedup": (nb_r / tb) / max(1e-9, np_r / tp), "nnc_batched": rb["num_new_configs"],
-                "nnc_plain": rpl["num_new_configs"], "granularity": cbase["granularity"]}
+                "nnc_plain": rpl["num_new_configs"], "granularity": cbase["granularity"],
+                "round_budget": args.batch_rounds}
             log(f"{robot} batched {nb_r / tb:.1f} plans/s vs plain {np_r / tp:.1f} plans/s "
-                f"({(nb_r / tb) / max(1e-9, np_r / tp):.2f}x) on {len(rf)} connectable shelf problems")
+                f"({(nb_r / tb) / max(1e-9, np_r / tp):.2f}x) on {len(rp)} shelf problems")
             g_r = cbase["granularity"]
-            for p, r, pa in zip(rf, rres, rpaths):
+            for p, r, pa in zip(rp, rres, rpaths):
                 if r["solved"]:
                     paths_to_check.append((robot, p["scene"], pa, g_r))
 
-        for p, r, pa in zip(feas, bres, bpaths):
+        for p, r, pa in zip(bprobs, bres, bpaths):
             if r["solved"]:
                 paths_to_check.append(("fetch", p["scene"], pa, 64))
 
@@ -413,14 +473,22 @@ def main():
         log(f"uneven mix: {nb} Baxter + {npd} Panda + {nf} Fetch")
         mprobs = workloads.c3_problems(nb, npd, nf, seed=0)
         # nnc 128 for every class: a problem that ends at max_iters costs
-        # max_iters x nnc tasks, and the mix has such problems
+        # max_iters x nnc tasks, and the mix has such problems.  The plain
+        # form runs the same settings, one problem per launch, so both forms
+        # must return the same results bit for bit.
         mst = [dict(num_new_configs=128, granularity=128, range=0.5, balance=1, tree_ratio=0.5,
                     dynamic_domain=1, dd_radius=4.0, dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000,
                     max_iters=3000, seed=1) for p in mprobs]
         t0 = time.perf_counter()
         mres, mpaths, mstats = P.solve_batch(mprobs, mst)
         t_mix = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], st) for p, st in zip(mprobs, mst)]
+        t_mix_plain = time.perf_counter() - t0
+        m_same = all(r["iters"] == r1["iters"] and r["start_tree_size"] == r1["start_tree_size"] and
+                     pa.tobytes() == pa1.tobytes() for r, pa, (r1, pa1) in zip(mres, mpaths, mplain))
         n_m = sum(r["solved"] for r in mres)
+        n_mp = sum(r[0]["solved"] for r in mplain)
         per_robot = {}
         for p, r in zip(mprobs, mres):
             d = per_robot.setdefault(p["robot"], {"problems": 0, "solved": 0})
@@ -428,9 +496,13 @@ def main():
             d["solved"] += r["solved"]
         out["mix"] = {"problems": len(mprobs), "solved": n_m, "plans_per_s": n_m / t_mix, "wall_s": t_mix,
                       "kernel_ms": mstats["kernel_ns"] / 1e6, "per_robot": per_robot,
+                      "solved_plain": n_mp, "plans_per_s_plain": n_mp / t_mix_plain, "plain_wall_s": t_mix_plain,
+                      "speedup": (n_m / t_mix) / max(1e-9, n_mp / t_mix_plain), "identical_results": bool(m_same),
                       "settings": "nnc 128, g 128, balance 1, ratio 0.5, dd on, range 0.5, max_iters 3000; one persistent "
-                                  "grid per robot class, all classes concurrently"}
-        log(f"mix: {n_m}/{len(mprobs)} solved in {t_mix:.3f} s ({n_m / t_mix:.1f} plans/s)")
+                                  "grid per robot class, all classes concurrently; plain: the same settings, one "
+                                  "problem per launch"}
+        log(f"mix: {n_m}/{len(mprobs)} solved in {t_mix:.3f} s ({n_m / t_mix:.1f} plans/s) vs plain "
+            f"{n_mp / t_mix_plain:.1f} plans/s ({out['mix']['speedup']:.2f}x), identical={m_same}")
         for p, r, pa in zip(mprobs, mres, mpaths):
             if r["solved"]:
                 paths_to_check.append((p["robot"], p["scene"], pa, 128))
@@ -441,7 +513,8 @@ def main():
         n_req = max(1, int(args.serve_rate * args.serve_seconds))
         log(f"serving: {n_req} requests at {args.serve_rate:.0f}/s")
         reqs = loadgen.request_pool(n_req)
-        sv = loadgen.run_stream(reqs, args.serve_rate, args.serve_seconds, young_rounds=args.serve_young)
+        sv = loadgen.run_stream(reqs, args.serve_rate, args.serve_seconds, young_rounds=args.serve_young,
+                                deadline_ms=args.serve_deadline_ms)
         outs = sv.pop("_outs")
         out["serving"] = sv
         log(f"serving: {sv['solved']}/{n_req} solved, p50 {sv['p50_ms']:.1f} ms, "
@@ -454,7 +527,18 @@ def main():
     if not args.skip_cpu:
         from oracle import oracle
         from prrtc_amd.robots import load_model
-        threads = min(16, os.cpu_count() or 1)
+        # the host's cores: nproc, and the ones this process may run on; a
+        # GPU box gives one GPU's job a 16-thread share of its CPUs
+        nproc = os.cpu_count() or 1
+        affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+        threads = max(1, min(args.cpu_threads, affinity))
+        build = oracle.use_fastest_build()
+        host_cpu = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                host_cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
         models = {}
 
         def model(robot):
@@ -466,12 +550,29 @@ def main():
         t0 = time.perf_counter()
         cpu_free = model("panda").validate_edges(scene, sample, g, threads=threads)
         dt = time.perf_counter() - t0
-        agree = float((cpu_free == free[: args.cpu_edges]).mean())
+        gpu_free = free[: args.cpu_edges]
+        agree = float((cpu_free == gpu_free).mean())
+        # verdict flips against the reference-rounding checker: each must have
+        # a pose within 1e-6 m of contact (distance from the oracle's exhaustive
+        # clearance over the edge's g poses)
+        flips = np.nonzero(cpu_free != gpu_free)[0]
+        near = 0
+        if len(flips):
+            fe = sample[flips]
+            delta = (fe[:, 1] - fe[:, 0]) / np.float32(g)
+            fq = (fe[:, 0][:, None, :] + np.arange(1, g + 1, dtype=np.float32)[None, :, None] * delta[:, None, :])
+            _, fm = model("panda").check(scene, fq.reshape(-1, fe.shape[-1]).astype(np.float32), margin=True)
+            near = int((fm.reshape(len(flips), g).min(axis=1) < 1e-6).sum())
         out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "edge checks/s", "cores": threads, "kind": "port",
-                               "sample": f"first {len(sample)} of the same 1M edges, g={g}, C oracle (-O3), "
-                                         f"{threads} threads, static partition",
-                               "seconds": dt, "verdict_agreement": agree}
-        log(f"cpu: {len(sample) / dt:.0f} edge checks/s on {threads} threads, agreement {agree:.4f}")
+                               "sample": f"first {len(sample)} of the same 1M edges, g={g}, C oracle (-O3, "
+                                         f"-march={build}), {threads} threads, static partition",
+                               "seconds": dt, "verdict_agreement": agree, "host_nproc": nproc,
+                               "host_cpus_available": affinity, "host_cpu": host_cpu}
+        out["parity"] = {"edges_compared": len(sample), "verdict_flips": int(len(flips)),
+                         "near_contact_flips": near, "flips_away_from_contact": int(len(flips)) - near,
+                         "checker": "oracle in the reference's rounding (separate multiplies and adds)"}
+        log(f"cpu: {len(sample) / dt:.0f} edge checks/s on {threads} threads ({build}; nproc {nproc}, "
+            f"{affinity} available), agreement {agree:.6f}, {len(flips)} flips ({near} near contact)")
         one = np.ascontiguousarray(edges[: max(1, args.cpu_edges // 25)])
         t0 = time.perf_counter()
         model("panda").validate_edges(scene, one, g, threads=1)
@@ -479,25 +580,69 @@ def main():
         out["cpu_baseline_1thread"] = {"value": len(one) / dt1, "unit": "edge checks/s", "cores": 1,
                                        "sample": f"first {len(one)} of the same edges", "seconds": dt1}
         log(f"cpu: {len(one) / dt1:.0f} edge checks/s on 1 thread")
+        from concurrent.futures import ThreadPoolExecutor
         if "c0" in out:
-            c0 = workloads.c0_problem(0)
-            times, solved = [], 0
-            for seed in range(8):
+            # the C0 problems through the CPU RRT-Connect, and through the
+            # oracle's planner in its sequential mode (each slot sees every
+            # earlier insertion at once, the extreme of the reference's
+            # concurrent visibility) against the GPU's round-synchronous plans
+            pm = model("panda")
+
+            def cpu_c0(ip):
+                i, p = ip
                 t0 = time.perf_counter()
-                rc, res, path = model("panda").rrt_connect(c0["scene"], c0["start"], c0["goals"][0], 32, 0.5,
-                                                           100000, seed)
-                times.append((time.perf_counter() - t0) * 1e3)
-                solved += res.solved
-            out["cpu_rrt_connect_c0"] = {"seeds": 8, "solved": solved, "p50_ms": pct(times, 50),
-                                         "p99_ms": pct(times, 99), "cores": 1,
-                                         "kind": "project CPU RRT-Connect (vamp unavailable offline)"}
-            log(f"cpu rrt-connect c0: p50 {out['cpu_rrt_connect_c0']['p50_ms']:.1f} ms")
+                res = pm.rrt_connect(p["scene"], p["start"], p["goals"][0], 32, 0.5, 100000, i)[1]
+                return (time.perf_counter() - t0) * 1e3, bool(res.solved), float(res.cost)
+
+            def seq_c0(ip):
+                i, p = ip
+                st = dict(c0_st, seed=i, max_iters=min(args.c0_rounds, 2000))
+                res = pm.plan(p["scene"], p["start"], p["goals"], st, mode=1)[1]
+                return bool(res.solved), float(res.cost), int(res.iters)
+
+            with ThreadPoolExecutor(threads) as ex:
+                rc_out = list(ex.map(cpu_c0, enumerate(c0_probs)))
+                seq_out = list(ex.map(seq_c0, enumerate(c0_probs)))
+            out["cpu_rrt_connect_c0"] = {"problems": len(c0_probs), "solved": sum(s for _, s, _ in rc_out),
+                                         "p50_ms": pct([t for t, _, _ in rc_out], 50),
+                                         "p99_ms": pct([t for t, _, _ in rc_out], 99),
+                                         "cost_p50": pct([c for _, s, c in rc_out if s], 50), "cores": 1,
+                                         "kind": "project CPU RRT-Connect (vamp unavailable offline), one problem "
+                                                 "per thread, each timed alone"}
+            gc = [r["cost"] for r in c0_res if r["solved"]]
+            sc_ = [c for s, c, _ in seq_out if s]
+            gi = [r["iters"] for r in c0_res if r["solved"]]
+            si = [it for s, _, it in seq_out if s]
+            dist = {"problems": len(c0_probs), "gpu_solved": len(gc), "sequential_solved": len(sc_),
+                    "gpu_cost_p10_p50_p90": [pct(gc, 10), pct(gc, 50), pct(gc, 90)],
+                    "sequential_cost_p10_p50_p90": [pct(sc_, 10), pct(sc_, 50), pct(sc_, 90)],
+                    "gpu_rounds_p50_p90": [pct(gi, 50), pct(gi, 90)],
+                    "sequential_rounds_p50_p90": [pct(si, 50), pct(si, 90)],
+                    "paired_cost_ratio_p50": pct([r["cost"] / c for r, (s, c, _) in zip(c0_res, seq_out)
+                                                  if r["solved"] and s and c > 0], 50),
+                    "reference_semantics": "oracle or_plan mode 1 (sequential visibility), same settings "
+                                           "(max_iters capped at 2000 on the CPU)"}
+            try:
+                from scipy import stats as sst
+                if len(gc) > 1 and len(sc_) > 1:
+                    dist["cost_mannwhitney_p"] = float(sst.mannwhitneyu(gc, sc_, alternative="two-sided").pvalue)
+                    dist["cost_ks_p"] = float(sst.ks_2samp(gc, sc_).pvalue)
+                    dist["rounds_mannwhitney_p"] = float(sst.mannwhitneyu(gi, si, alternative="two-sided").pvalue)
+                dist["success_fisher_p"] = float(sst.fisher_exact(
+                    [[len(gc), len(c0_probs) - len(gc)], [len(sc_), len(c0_probs) - len(sc_)]])[1])
+                dist["test"] = ("two-sided Mann-Whitney U and Kolmogorov-Smirnov on the solved problems' costs "
+                                "and rounds; Fisher's exact test on success counts")
+            except ImportError:
+                dist["test"] = "scipy unavailable"
+            out["c0_vs_sequential"] = dist
+            log(f"c0 distributions: GPU cost p50 {dist['gpu_cost_p10_p50_p90'][1]:.3f} vs sequential "
+                f"{dist['sequential_cost_p10_p50_p90'][1]:.3f} (MW p {dist.get('cost_mannwhitney_p', float('nan')):.3f}), "
+                f"solved {len(gc)} vs {len(sc_)}")
         if "batched" in out:
-            # the same connectable Fetch problems through the CPU RRT-Connect,
-            # one problem per host thread (ctypes releases the GIL), bounded
-            # at 5000 samples per problem
-            from concurrent.futures import ThreadPoolExecutor
-            fprobs = [p for p, r in zip(bprobs, pres) if r["solved"]]
+            # the Fetch problems the batch connected through the CPU
+            # RRT-Connect, one problem per host thread (ctypes releases the
+            # GIL), bounded at 5000 samples per problem
+            fprobs = feas
             fr = model("fetch")
 
             def cpu_solve(p):
@@ -510,12 +655,12 @@ def main():
             dt = time.perf_counter() - t0
             cpu_solved = sum(sv for sv, _ in cpu_res)
             # path cost against the GPU planner's on the problems both solved
-            # (the same connectable set, in order)
-            both = [(g["cost"], c) for g, (sv, c) in zip(bres, cpu_res) if sv and g["solved"]]
+            fgpu = [r for r in bres if r["solved"]]
+            both = [(gr["cost"], c) for gr, (sv, c) in zip(fgpu, cpu_res) if sv]
             out["cpu_rrt_connect_c2"] = {"problems": len(fprobs), "solved": int(cpu_solved),
                                          "cost_p50": pct([c for sv, c in cpu_res if sv], 50),
                                          "cost_p90": pct([c for sv, c in cpu_res if sv], 90),
-                                         "gpu_over_cpu_cost_p50": pct([g / c for g, c in both if c > 0], 50),
+                                         "gpu_over_cpu_cost_p50": pct([a / c for a, c in both if c > 0], 50),
                                          "both_solved": len(both),
                                          "plans_per_s": cpu_solved / dt, "wall_s": dt, "cores": threads,
                                          "kind": "project CPU RRT-Connect (vamp unavailable offline)",

@@ -83,6 +83,7 @@ typedef struct prrtc_result {
     int64_t kernel_ns;        /* device time of the solve (pRRTC.cu:696-706) */
     int64_t copy_ns;          /* host<->device copies (pRRTC.cu:685-694, 709-713) */
     int64_t edge_checks;      /* edges whose verdict was decided */
+    int64_t pose_checks;      /* poses put on lanes for those edges (one pose per lane and pass) */
 } prrtc_result;
 
 /* Fill `s` with the reference's defaults (pRRTC_settings.hh:4-16), seed 1. */
@@ -151,6 +152,7 @@ typedef struct prrtc_batch_stats {
     int64_t edge_checks;      /* over all problems */
     int32_t solved;           /* problems solved */
     int32_t rounds;           /* planner rounds executed (max over problems) */
+    int64_t pose_checks;      /* over all problems */
 } prrtc_batch_stats;
 
 /* settings[i] applies to problem i (settings may also point to one struct

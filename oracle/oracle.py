@@ -62,10 +62,36 @@ class Result(C.Structure):
     ]
 
 
+LIB_V4_PATH = os.path.join(HERE, "liboracle_v4.so")
+
+
 def build(force: bool = False) -> None:
     src = os.path.join(HERE, "oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["make", "-s", "-B", "-C", HERE, "liboracle.so"])
+    stale = [p for p in (LIB_PATH, LIB_V4_PATH)
+             if force or not os.path.exists(p) or os.path.getmtime(p) < os.path.getmtime(src)]
+    if stale:
+        subprocess.check_call(["make", "-s", "-B", "-C", HERE] + [os.path.basename(p) for p in stale])
+
+
+def host_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return " avx512f " in line + " "
+    except OSError:
+        pass
+    return False
+
+
+def use_fastest_build() -> str:
+    """Load the AVX-512 build when this host has AVX-512 (the CPU baseline;
+    same source and rounding), else the portable one.  Call before any other
+    oracle function; returns the build's name."""
+    global LIB_PATH
+    if _lib is None and host_has_avx512() and os.path.exists(LIB_V4_PATH):
+        LIB_PATH = LIB_V4_PATH
+    return "x86-64-v4" if LIB_PATH == LIB_V4_PATH else "x86-64-v3"
 
 
 _lib = None

@@ -104,3 +104,17 @@ def pose_flops(robot: str, scene, centres: np.ndarray) -> np.ndarray:
         if not chk["direct"]:
             flops += np.where(hit, W["self"] * len(chk["pairs"]), 0.0)
     return flops
+
+
+def gate_pass_flops(robot: str, scene) -> float:
+    """Algorithmic work per pose of a planner validation pass, as an
+    estimate: the reference's FK plus every env gate sphere tested against
+    every obstacle of the scene (utils.cuh:211-242 for each gate).  Leaves
+    under hit gates and self-collision pairs are not counted, and the gate
+    loops' early exits at a hit are not taken off; the planner's colliding
+    passes stop before most of this work, its free passes do more."""
+    per_obstacle = sum(len(getattr(scene, t)) * w for t, w in zip(TABLES, TABLE_W))
+    if robot == "sphere":
+        return float(per_obstacle)
+    m = load_model(robot)
+    return float(F_FK[robot] + len(m["env_checks"]) * per_obstacle)

@@ -178,6 +178,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx
     const float dist = sqrtf(bsq);
     const float scale = fminf(1.0f, d.range / dist);
     int flags = 0, n_ok = 0, nn_o = 0;
+    int poses = 0;   // poses put on lanes (the same on every wave of the team)
     if (d.dd && d.radii[(size_t)t * d.cap + nn] < dist) {          // pRRTC.cu:420-427
         flags = kSkip;
     } else {
@@ -221,6 +222,7 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx
             ctx.dead = dead;
             ctx.key = key;
             ctx.pass = npass;
+            poses += __builtin_popcountll(ballot(!dead));
             // the other parity's word is next read in the following pass,
             // after this pass's closing barrier
             if (tid == 0 && tm.waves > 1) stopw[(npass + 1) & 1] = 0x7fffffff;
@@ -296,6 +298,8 @@ __device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx
         meta[3] = n_ok;
         meta[4] = nn_o;
         __hip_atomic_fetch_add(&st->edge_checks, (unsigned long long)edges, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&st->pose_checks, (unsigned long long)poses, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -1232,6 +1236,7 @@ static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int 
         r.start_tree_size = ps.size[0];
         r.goal_tree_size = ps.size[1];
         r.edge_checks = (int64_t)ps.edge_checks;
+        r.pose_checks = (int64_t)ps.pose_checks;
         *rounds = std::max(*rounds, ps.round);
         r.solved = ps.status == kStatusSolved;
         r.status = r.solved ? PRRTC_OK : (ps.status == kStatusPathLimit ? PRRTC_EOVERFLOW : PRRTC_OK);
@@ -1300,18 +1305,20 @@ static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, fl
     for (GroupRun &gr : runs) finish_group(gr, results, paths, path_cap, path_stride, &kernel_ns, &rounds);
     const int64_t wall = now_ns() - t0;
     int solved = 0;
-    int64_t edges = 0;
+    int64_t edges = 0, poses = 0;
     for (auto &hp : probs) {
         results[hp.index].wall_ns = wall;
         results[hp.index].kernel_ns = kernel_ns;
         results[hp.index].copy_ns = 0;
         solved += results[hp.index].solved;
         edges += results[hp.index].edge_checks;
+        poses += results[hp.index].pose_checks;
     }
     if (stats) {
         stats->wall_ns = wall;
         stats->kernel_ns = kernel_ns;
         stats->edge_checks = edges;
+        stats->pose_checks = poses;
         stats->solved = solved;
         stats->rounds = rounds;
     }

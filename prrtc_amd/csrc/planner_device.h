@@ -65,6 +65,7 @@ struct PlanState {
     float cost;
     int pad0;
     unsigned long long edge_checks;
+    unsigned long long pose_checks;   // lanes carrying a pose at the start of a pass, summed over passes
 };
 
 // Launch-wide control block.

@@ -428,6 +428,7 @@ void ServerImpl::harvest(RobotClass &c, std::vector<std::pair<uint64_t, Finished
         r.start_tree_size = ps.size[0];
         r.goal_tree_size = ps.size[1];
         r.edge_checks = (int64_t)ps.edge_checks;
+        r.pose_checks = (int64_t)ps.pose_checks;
         r.solved = ps.status == kStatusSolved;
         r.status = r.solved                          ? PRRTC_OK
                    : ps.status == kStatusPathLimit ? PRRTC_EOVERFLOW

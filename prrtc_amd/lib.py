@@ -51,7 +51,7 @@ class CResult(C.Structure):
         ("solved", C.c_int32), ("start_tree_size", C.c_int32), ("goal_tree_size", C.c_int32),
         ("path_length", C.c_int32), ("iters", C.c_int32), ("cost", C.c_float), ("path_nodes", C.c_int32),
         ("status", C.c_int32), ("wall_ns", C.c_int64), ("kernel_ns", C.c_int64), ("copy_ns", C.c_int64),
-        ("edge_checks", C.c_int64),
+        ("edge_checks", C.c_int64), ("pose_checks", C.c_int64),
     ]
 
 
@@ -64,7 +64,7 @@ class CProblem(C.Structure):
 class CBatchStats(C.Structure):
     _fields_ = [
         ("wall_ns", C.c_int64), ("kernel_ns", C.c_int64), ("edge_checks", C.c_int64), ("solved", C.c_int32),
-        ("rounds", C.c_int32),
+        ("rounds", C.c_int32), ("pose_checks", C.c_int64),
     ]
 
 

@@ -117,3 +117,25 @@ def test_cuboid_cull_radius():
         R = float(np.linalg.norm(h.astype(np.float64)))
         assert R + 1e-3 <= cull * (1 + 1e-7)
         assert cull <= R * (1 + 2.0 ** -16) + 1e-3 + 1e-6
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror in prrtc_amd/lib.py has the size and field offsets
+    the C compiler gives the header's struct (a mismatch garbles every
+    result after the first of a batch)."""
+    import subprocess
+    import prrtc_amd.lib as L
+    exe = str(tmp_path / "abi_sizes")
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "abi_sizes.c"), "-o", exe], check=True)
+    got = dict(l.split() for l in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines())
+    mirrors = {"prrtc_scene": L.CScene, "prrtc_settings": L.CSettings, "prrtc_result": L.CResult,
+               "prrtc_problem": L.CProblem, "prrtc_batch_stats": L.CBatchStats, "prrtc_tuning": L.CTuning,
+               "prrtc_server_config": L.CServerConfig, "prrtc_server_stats": L.CServerStats,
+               "prrtc_request_timing": L.CRequestTiming}
+    for name, cls in mirrors.items():
+        assert int(got[name]) == C.sizeof(cls), name
+    for key, v in got.items():
+        if "." in key:
+            name, field = key.split(".")
+            assert getattr(mirrors[name], field).offset == int(v), key
