@@ -140,15 +140,35 @@ def test_server_deadline(gpu, oracle_robots):
     assert 0.06 <= waited < 2.0
     assert_same(r1, path1, *alone, "beside a timed-out request")
     assert stats["timed_out"] == 1 and stats["completed"] == 2
-    # a budget that holds one sealed problem: the second waits in the queue
-    # until its deadline and is dropped unadmitted
+    # a budget that holds one sealed problem: the second waits in the queue;
+    # whether it is dropped there or admitted once the first has stopped,
+    # neither outlives its deadline by more than a few slices
     one = int(1.5 * 2 * 20000 * 5 * 4)
     with gpu.Server(slice_us=500, deadline_ms=40, budget_bytes=one) as srv:
         ta, tb = srv.submit(sealed, sst), srv.submit(sealed, sst)
         ra, _, tma = srv.poll(ta)
         rb, _, tmb = srv.poll(tb)
-    assert ra["status"] == rb["status"] == PRRTC_ETIMEDOUT
-    assert tmb["slices"] == 0 and tmb["resident_ns"] == 0 and tmb["queued_ns"] >= 40e6
+        stats = srv.stats()
+    assert ra["status"] == rb["status"] == PRRTC_ETIMEDOUT and stats["timed_out"] == 2
+    for tm in (tma, tmb):
+        assert 40e6 <= tm["queued_ns"] + tm["resident_ns"] <= 40e6 + 30e6, tm
+    assert tmb["queued_ns"] > 0 and stats["peak_resident_bytes"] <= one
+
+
+def test_server_cancel_queued(gpu):
+    """A request queued behind a full budget and cancelled there ends at
+    once, never admitted (no slice, no resident time), status ECANCELED."""
+    from prrtc_amd.lib import PRRTC_ECANCELED
+    sealed, sst = _sealed_request()
+    one = int(1.5 * 2 * 20000 * 5 * 4)
+    with gpu.Server(slice_us=500, budget_bytes=one) as srv:
+        ta, tb = srv.submit(sealed, sst), srv.submit(sealed, sst)
+        srv.cancel(tb)
+        rb, _, tmb = srv.poll(tb)
+        srv.cancel(ta)
+        ra, _, _ = srv.poll(ta)
+    assert rb["status"] == ra["status"] == PRRTC_ECANCELED
+    assert tmb["slices"] == 0 and tmb["resident_ns"] == 0
 
 
 def test_server_cancel(gpu, oracle_robots):
