@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plans", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--profile-plan", action="store_true",
+                    help="only the C2 Fetch batch (one persistent k_plan launch after one warm-up batch), for "
+                         "rocprofv3 runs of the planner kernel")
     ap.add_argument("--profile-run", action="store_true",
                     help="only the free-set validation launches and one 256 MiB device copy (for rocprofv3 "
                          "counter runs: the copy's counters scale FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -127,6 +130,20 @@ def main():
     for robot, t in tuned.get("launch", {}).items():
         P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0),
                      t.get("crowd_above", 0))
+
+    if args.profile_plan:
+        bprobs = workloads.c2_problems(args.batch, seed=0)
+        nnc_b = tuned.get("batch_nnc", {}).get("fetch", 128)
+        bst = dict(granularity=64, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=0, max_samples=100000,
+                   seed=1, num_new_configs=nnc_b, max_iters=args.batch_rounds)
+        P.solve_batch(bprobs[:8], dict(bst, max_iters=50))
+        res, _, stats = P.solve_batch(bprobs, bst)
+        flops = sum(r["pose_checks"] * accounting.gate_pass_flops("fetch", p["scene"]) for p, r in zip(bprobs, res))
+        print(json.dumps({"profile_plan": True, "problems": len(bprobs), "solved": stats["solved"],
+                          "kernel_ms": stats["kernel_ns"] / 1e6, "rounds": stats["rounds"],
+                          "pose_checks": stats["pose_checks"], "edge_checks": stats["edge_checks"],
+                          "estimated_flops": flops}), flush=True)
+        return
 
     # ---------------------------------------------------------------- inputs
     log("building the scene and the edges")
@@ -156,7 +173,8 @@ def main():
     # -------------------------------------------------------------- roofline
     free_edges = np.ascontiguousarray(edges[free])
     n_free = len(free_edges)
-    reps = max(1, (1 << 17) // max(1, n_free))
+    # the free set repeated (rounded up) and cut to exactly 131,072 edges
+    reps = max(1, -(-(1 << 17) // max(1, n_free)))
     roof_edges = np.ascontiguousarray(np.tile(free_edges, (reps, 1, 1))[: 1 << 17]) if n_free else free_edges
     n_roof = len(roof_edges)
     d_roof = torch.from_numpy(roof_edges).to(dev)
