@@ -451,36 +451,41 @@ def main():
 
         # ------------------------------ batched vs plain, Panda and Baxter
         # The same comparison on the shelf problem sets of the uneven mix,
-        # one class at a time, with each class's settings and tuned nnc,
-        # both forms under one round budget.
+        # one class at a time, with each class's settings and tuned nnc:
+        # on the problems a pilot batch connects (max_iters 20000 in both
+        # forms), and on every problem under the round budget in both forms.
         from prrtc_amd.autotune import SETTINGS as CLASS_SETTINGS, problems as class_problems
         out["batched_per_robot"] = {}
         for robot in ("panda", "baxter"):
             rp = class_problems(robot, args.robot_batch)
             cbase = CLASS_SETTINGS[robot]
-            rb = dict(cbase, num_new_configs=tuned.get("batch_nnc", {}).get(robot, 128), max_iters=args.batch_rounds)
-            rpl = dict(cbase, num_new_configs=tuned.get("plain_nnc", {}).get(robot, 128),
-                       max_iters=args.batch_rounds)
-            t0 = time.perf_counter()
-            rres, rpaths, _ = P.solve_batch(rp, rb)
-            tb = time.perf_counter() - t0
-            t0 = time.perf_counter()
-            rplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"], rpl) for p in rp]
-            tp = time.perf_counter() - t0
-            nb_r = sum(r["solved"] for r in rres)
-            np_r = sum(r[0]["solved"] for r in rplain)
-            out["batched_per_robot"][robot] = {
-                "problems": len(rp), "solved_batched": nb_r, "solved_plain": np_r,
-                "plans_per_s_batched": nb_r / tb, "plans_per_s_plain": np_r / tp,
-                "speedup": (nb_r / tb) / max(1e-9, np_r / tp), "nnc_batched": rb["num_new_configs"],
-                "nnc_plain": rpl["num_new_configs"], "granularity": cbase["granularity"],
-                "round_budget": args.batch_rounds}
-            log(f"{robot} batched {nb_r / tb:.1f} plans/s vs plain {np_r / tp:.1f} plans/s "
-                f"({(nb_r / tb) / max(1e-9, np_r / tp):.2f}x) on {len(rp)} shelf problems")
-            g_r = cbase["granularity"]
-            for p, r, pa in zip(rp, rres, rpaths):
-                if r["solved"]:
-                    paths_to_check.append((robot, p["scene"], pa, g_r))
+            nb_nnc = tuned.get("batch_nnc", {}).get(robot, 128)
+            np_nnc = tuned.get("plain_nnc", {}).get(robot, 128)
+            row = {"problems": len(rp), "nnc_batched": nb_nnc, "nnc_plain": np_nnc,
+                   "granularity": cbase["granularity"]}
+            pilot, _, _ = P.solve_batch(rp, dict(cbase, num_new_configs=64, max_iters=2000))
+            rf = [p for p, r in zip(rp, pilot) if r["solved"]]
+            for key, probs_, rounds in (("connectable", rf, 20000), ("all_budgeted", rp, args.batch_rounds)):
+                t0 = time.perf_counter()
+                rres, rpaths, _ = P.solve_batch(probs_, dict(cbase, num_new_configs=nb_nnc, max_iters=rounds))
+                tb = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                rplain = [P.solve(p["robot"], p["start"], p["goals"], p["scene"],
+                                  dict(cbase, num_new_configs=np_nnc, max_iters=rounds)) for p in probs_]
+                tp = time.perf_counter() - t0
+                nb_r = sum(r["solved"] for r in rres)
+                np_r = sum(r[0]["solved"] for r in rplain)
+                row[key] = {"problems": len(probs_), "max_iters": rounds, "solved_batched": nb_r,
+                            "solved_plain": np_r, "plans_per_s_batched": nb_r / tb,
+                            "plans_per_s_plain": np_r / tp, "speedup": (nb_r / tb) / max(1e-9, np_r / tp)}
+                log(f"{robot} {key}: batched {nb_r / tb:.1f} plans/s vs plain {np_r / tp:.1f} plans/s "
+                    f"({row[key]['speedup']:.2f}x) on {len(probs_)} shelf problems")
+                if key == "connectable":
+                    for p, r, pa in zip(probs_, rres, rpaths):
+                        if r["solved"]:
+                            paths_to_check.append((robot, p["scene"], pa, cbase["granularity"]))
+            row["speedup"] = row["connectable"]["speedup"]
+            out["batched_per_robot"][robot] = row
 
         for p, r, pa in zip(bprobs, bres, bpaths):
             if r["solved"]:

@@ -104,8 +104,11 @@ typedef struct {
 /* pRRTC planner restated on the CPU (pRRTC.cu:305-602, 607-766).
  * mode 0: round-synchronous -- every slot of a round samples against the trees
  *         as they were at the round start; new nodes are appended in slot order.
- *         (The HIP planner implements exactly this.)  unpinned vs the reference
- *         planner (which cannot run here); pinned against the HIP planner.
+ *         (The HIP planner implements exactly this.)  With one slot both modes
+ *         are the reference planner's deterministic case (one block, unshuffled
+ *         bases, pRRTC.cu:61-65), pinned by hand-derived known answers
+ *         (tests/planner_known.py); with more slots the reference depends on
+ *         inter-block timing, and only distributions can be compared.
  * mode 1: sequential -- slots take turns, each seeing every earlier insertion
  *         at once, the extreme of the reference's concurrent visibility.
  * path_out holds up to path_cap rows of dof floats. */
