@@ -136,6 +136,21 @@ int prrtc_nearest_async(int dof, const float *d_trees, int cap, int n_nodes, int
 int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, const prrtc_scene *scene,
                 const prrtc_settings *settings, prrtc_result *result, float *path, int path_cap);
 
+/* prrtc_solve without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * All device state of the solve (trees, slot states, staged chains, path
+ * rows) lives in one static device array of PRRTC_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends; the call runs on the null stream and
+ * returns when done, with the result prrtc_solve returns, bit for bit.  A
+ * problem whose state does not fit (mostly max_samples and path_cap) fails
+ * with PRRTC_ENOMEM and a message naming the sizes.  Calls on one GPU share
+ * the array: calls that overlap there (several host threads) race, so make
+ * at most one at a time per GPU. */
+#define PRRTC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int prrtc_solve_noalloc(int robot, const float *start, const float *goals, int n_goals, const prrtc_scene *scene,
+                        const prrtc_settings *settings, prrtc_result *result, float *path, int path_cap);
+
 /* Many independent problems in one persistent launch (no reference
  * counterpart; the reference solves one problem per launch). */
 typedef struct prrtc_problem {

@@ -125,6 +125,8 @@ SIGNATURES = [
                                     C.POINTER(CRequestTiming)]),
     ("prrtc_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(CServerStats)]),
     ("prrtc_server_cancel", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("prrtc_solve_noalloc", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(CScene),
+                                      C.POINTER(CSettings), C.POINTER(CResult), C.c_void_p, C.c_int]),
     ("prrtc_server_destroy", C.c_int, [C.c_void_p]),
 ]
 
@@ -282,6 +284,23 @@ def solve(robot, start, goals, scene: Scene, settings: dict | None = None, path_
     rc = load().prrtc_solve(rid, start.ctypes.data, goals.ctypes.data, len(goals), C.byref(cs), C.byref(st),
                             C.byref(res), path.ctypes.data, path_cap)
     _check(rc, "prrtc_solve")
+    return _result_dict(res), path[:res.path_nodes].copy()
+
+
+def solve_noalloc(robot, start, goals, scene: Scene, settings: dict | None = None, path_cap: int = 8192):
+    """prrtc_solve_noalloc: solve() with its device state in the library's
+    static scratch array (no device or pinned allocation)."""
+    rid = robot_id(robot)
+    D = load().prrtc_robot_dof(rid)
+    start = _f(start).reshape(D)
+    goals = _f(goals).reshape(-1, D)
+    st = c_settings(settings or {})
+    cs = c_scene(scene)
+    res = CResult()
+    path = np.zeros((path_cap, D), dtype=np.float32)
+    rc = load().prrtc_solve_noalloc(rid, start.ctypes.data, goals.ctypes.data, len(goals), C.byref(cs), C.byref(st),
+                                    C.byref(res), path.ctypes.data, path_cap)
+    _check(rc, "prrtc_solve_noalloc")
     return _result_dict(res), path[:res.path_nodes].copy()
 
 
