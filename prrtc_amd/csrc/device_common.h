@@ -145,9 +145,14 @@ __device__ __forceinline__ float4 ld_obs(const float4 *p) {
 // itself; cuboid, capsule: the cull balls below) stays outside that ball on
 // every lane is at least 1 mm from every one of the N spheres, so all N of
 // its tests are positive and the wave skips the obstacle with one test.
-template <int N, bool EARLY, bool LOCAL = false, bool GROUP = false>
+//
+// ANY: only whether some of the N spheres hits is wanted (a leaf group, whose
+// hits the caller ORs): one running minimum over all N spheres replaces the N
+// per-sphere minima (some value < 0 iff their minimum is), which frees N - 1
+// registers of every leaf group; hit[0] carries the answer.
+template <int N, bool EARLY, bool LOCAL = false, bool GROUP = false, bool ANY = false>
 __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N], const float (&y)[N],
-                                          const float (&z)[N], const float (&r)[N], bool (&hit)[N],
+                                          const float (&z)[N], const float (&r)[N], bool (&hit)[ANY ? 1 : N],
                                           float gx = 0.0f, float gy = 0.0f, float gz = 0.0f, float gr = 0.0f) {
 #pragma clang fp contract(off)
     auto group_far = [&](float cx, float cy, float cz, float cr) {
@@ -158,18 +163,19 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
     // value < 0 (a collision) iff the minimum is < 0.  One v_min per test
     // replaces a compare, a select and the bit packing of a boolean OR; a
     // NaN test value is dropped by fminf as it would fail `< 0`.
-    float m[N];
+    constexpr int M = ANY ? 1 : N;   // running minima kept
+    float m[M];
 #pragma unroll
-    for (int n = 0; n < N; n++) m[n] = __builtin_huge_valf();
+    for (int n = 0; n < M; n++) m[n] = __builtin_huge_valf();
     auto done = [&]() {
         bool any_hit = false;
 #pragma unroll
-        for (int n = 0; n < N; n++) any_hit |= m[n] < 0.0f;
+        for (int n = 0; n < M; n++) any_hit |= m[n] < 0.0f;
         return ballot(!any_hit) == 0;
     };
     auto finish = [&]() {
 #pragma unroll
-        for (int n = 0; n < N; n++) hit[n] = m[n] < 0.0f;
+        for (int n = 0; n < M; n++) hit[n] = m[n] < 0.0f;
     };
     for (int i = 0; i < o.n_sph; i++) {                       // utils.cuh:58-81
         const float4 s = ld_obs<LOCAL>(o.sph + i);
@@ -177,7 +183,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
 #pragma unroll
         for (int n = 0; n < N; n++) {
             const float rs = s.w + r[n];
-            m[n] = fminf(m[n], fmaf(-rs, rs, sq3(s.x - x[n], s.y - y[n], s.z - z[n])));
+            m[ANY ? 0 : n] = fminf(m[ANY ? 0 : n], fmaf(-rs, rs, sq3(s.x - x[n], s.y - y[n], s.z - z[n])));
         }
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
@@ -199,7 +205,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
             const float rs = r[n] + p.w;
             const float d = sq3(x[n] - fmaf(v.x, cdf, p.x), y[n] - fmaf(v.y, cdf, p.y), z[n] - fmaf(v.z, cdf, p.z));
-            m[n] = fminf(m[n], fmaf(-rs, rs, d));
+            m[ANY ? 0 : n] = fminf(m[ANY ? 0 : n], fmaf(-rs, rs, d));
         }
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
@@ -212,7 +218,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float cdf = fmaxf(fminf(dot * v.w, 1.0f), 0.0f);
             const float rs = r[n] + p.w;
             const float d = sq3(x[n] - p.x, y[n] - p.y, z[n] - fmaf(v.z, cdf, p.z));
-            m[n] = fminf(m[n], fmaf(-rs, rs, d));
+            m[ANY ? 0 : n] = fminf(m[ANY ? 0 : n], fmaf(-rs, rs, d));
         }
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
@@ -234,7 +240,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, fmaf(a1.y, ys, a1.z * zs))) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, fmaf(a2.y, ys, a2.z * zs))) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(fmaf(a3.x, xs, fmaf(a3.y, ys, a3.z * zs))) - a2.w);
-            m[n] = fminf(m[n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
+            m[ANY ? 0 : n] = fminf(m[ANY ? 0 : n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
         }
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
@@ -252,7 +258,7 @@ __device__ __forceinline__ void env_multi(const Obstacles &o, const float (&x)[N
             const float d1 = fmaxf(0.0f, fabsf(fmaf(a1.x, xs, a1.y * ys)) - c.w);
             const float d2 = fmaxf(0.0f, fabsf(fmaf(a2.x, xs, a2.y * ys)) - a1.w);
             const float d3 = fmaxf(0.0f, fabsf(zs) - a2.w);
-            m[n] = fminf(m[n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
+            m[ANY ? 0 : n] = fminf(m[ANY ? 0 : n], fmaf(-r[n], r[n], sq3(d1, d2, d3)));
         }
         if (EARLY && (i & 7) == 7 && done()) return finish();
     }
@@ -294,16 +300,26 @@ struct CheckCtx {
                                               const float (&r)[N], bool (&hit)[N]) const {
         prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
     }
-    // The N spheres lie inside the ball (gx, gy, gz; gr) on every lane.  The
-    // one-wave-per-edge kernels ignore the bound: with it the Panda kernel
-    // needs 256 VGPRs plus AGPRs instead of 221 (one wave per SIMD instead
-    // of two), which costs more than the skipped tests save; the planner's
-    // tasks use it (TaskCtx).
+    // Whether any of the N spheres hits an obstacle (a leaf group).
     template <int N, bool EARLY>
-    __device__ __forceinline__ void env_multi_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
-                                                      const float (&r)[N], bool (&hit)[N], float, float, float,
-                                                      float) const {
-        prrtc::env_multi<N, EARLY>(obs, x, y, z, r, hit);
+    __device__ __forceinline__ bool env_any(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                            const float (&r)[N]) const {
+        bool h[1];
+        prrtc::env_multi<N, EARLY, false, false, true>(obs, x, y, z, r, h);
+        return h[0];
+    }
+    // The same for N spheres that lie inside the ball (gx, gy, gz; gr) on
+    // every lane: an obstacle whose bounding ball stays outside it is skipped
+    // for the whole group (GROUP).  With one running minimum per group the
+    // bound fits the one-wave-per-edge kernels' register budget (Panda 209
+    // VGPRs, Fetch 255: two waves per SIMD, no scratch).
+    template <int N, bool EARLY>
+    __device__ __forceinline__ bool env_any_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                                    const float (&r)[N], float gx, float gy, float gz,
+                                                    float gr) const {
+        bool h[1];
+        prrtc::env_multi<N, EARLY, false, true, true>(obs, x, y, z, r, h, gx, gy, gz, gr);
+        return h[0];
     }
     __device__ __forceinline__ bool self(float ax, float ay, float az, float bx, float by, float bz, float rs) const {
         return self_hit(ax, ay, az, bx, by, bz, rs);

@@ -90,10 +90,19 @@ struct TaskCtx : CheckCtx {
         prrtc::env_multi<N, EARLY, true>(obs, x, y, z, r, hit);
     }
     template <int N, bool EARLY>
-    __device__ __forceinline__ void env_multi_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
-                                                      const float (&r)[N], bool (&hit)[N], float gx, float gy,
-                                                      float gz, float gr) const {
-        prrtc::env_multi<N, EARLY, true, true>(obs, x, y, z, r, hit, gx, gy, gz, gr);
+    __device__ __forceinline__ bool env_any(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                            const float (&r)[N]) const {
+        bool h[1];
+        prrtc::env_multi<N, EARLY, true, false, true>(obs, x, y, z, r, h);
+        return h[0];
+    }
+    template <int N, bool EARLY>
+    __device__ __forceinline__ bool env_any_bounded(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                                    const float (&r)[N], float gx, float gy, float gz,
+                                                    float gr) const {
+        bool h[1];
+        prrtc::env_multi<N, EARLY, true, true, true>(obs, x, y, z, r, h, gx, gy, gz, gr);
+        return h[0];
     }
     // CheckCtx::stop in path mode (a planner task is never in pose mode),
     // with the stop key shared across waves.

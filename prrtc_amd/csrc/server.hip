@@ -347,7 +347,10 @@ void ServerImpl::launch(RobotClass &c) {
         // tasks, so no round is left with all its slots claimed
         c.h_mask[e >> 6] |= 1ull << (e & 63);
         c.h_mask[W + (e >> 6)] |= 1ull << (e & 63);
-        if (rq->round >= cfg.young_rounds) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
+        // a stopped request counts as young: its slots (skipped tasks) are
+        // claimed first, so the round that ends it commits at once instead of
+        // waiting behind the young requests of a busy server
+        if (rq->round >= cfg.young_rounds && !rq->stop_reason) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
         lds_f4 = std::max(lds_f4, (rq->lay.scene_floats - 8) / 4);
         ext = std::max(ext, rq->lay.max_ext);
         max_nnc = std::max(max_nnc, rq->st.num_new_configs);

@@ -120,7 +120,7 @@ def test_group_bounds_hold_every_leaf(oracle_robots):
         r = np.array([s["r"] for s in m["spheres"]])
         bounded = set()
         for label, body in blocks:
-            mb = re.search(r"env_multi_bounded<\d+, true>\(.*, (-?[0-9.e+-]+)f\);", body)
+            mb = re.search(r"env_any_bounded<\d+, true>\(.*, (-?[0-9.e+-]+)f\);", body)
             if not mb:
                 continue
             bounded.add(label)
