@@ -543,18 +543,28 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             int p_out = -1, s_out = -1, n_out = 1, hot = 0, crowd = 0;   // p_out < 0: the workgroup leaves
             // Claim up to k consecutive slots of problem q if q runs and has
             // unclaimed slots in this round; returns the first slot (-1: none)
-            // and their number in `got`.  Plain loads come first, so the
-            // counters' cache lines stay cheap to share; a claim counts only
-            // below nnc and finished problems are never incremented, so it
-            // cannot wrap.  The claim of the last slot closes the problem in
-            // the open mask (before that task can finish, so before the
-            // commit reopens it).
-            auto try_claim = [&](int q, int k, int &got) -> int {
+            // and their number in `got`.  A round the mask scan saw open is
+            // claimed with the atomic alone: the counter is below nnc only
+            // in an open round of a running problem (the commit resets it
+            // for those only, after the round's state is written), and an
+            // increment of a closed round's or a finished problem's counter
+            // changes nothing (the next commit resets the former; the latter
+            // holds 2^30, and only the short window between a last claim
+            // or a final commit and its mask update sees a stale open bit).
+            // A round not seen open is checked with plain loads first, so a
+            // visit of a closed round costs no atomic.  The claim of the
+            // last slot closes the problem in the open mask (before that
+            // task can finish, so before the commit reopens it).
+            auto try_claim = [&](int q, int k, bool seen_open, int &got) -> int {
                 PlanState *st = states + q;
                 const int nnc = descs[q].nnc;
-                const bool live = __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
-                const bool open = live && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                                              &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
+                bool open = seen_open;
+                if (!open) {
+                    const bool live =
+                        __builtin_amdgcn_readfirstlane(load_acquire(&st->status)) == kStatusRunning;
+                    open = live && __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                                       &st->next_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < nnc;
+                }
                 const int s = open ? wave_atomic_add(&st->next_slot, k) : nnc;
                 if (s < 0 || s >= nnc) return -1;
                 got = min(k, nnc - s);
@@ -567,7 +577,15 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // running count and the next open problem from the cursor on
                 // (young ones first on a server); closed and finished
                 // problems are skipped through the masks, not visited
-                const MaskScan ms = scan_masks(ctrl, words, cursor);
+                // (and whether the furthest problem of a batch has an open
+                // round: its hot_key is read first, its bit checked in the scan)
+                const unsigned long long hk =
+                    prio_policy == kPrioOldest
+                        ? __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0ull;
+                const int hp = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
+                const int ph = hp >= 0 && hp < P ? hp : -1;
+                const MaskScan ms = scan_masks(ctrl, words, cursor, ph);
                 // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
                 // share their counters with the whole grid
@@ -582,13 +600,8 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 const int k = crowd_above > 0 && ms.running > crowd_above ? W : 1;
                 int got = 1;
                 // a batch serves the problem furthest along first, on all waves
-                const unsigned long long hk =
-                    prio_policy == kPrioOldest
-                        ? __hip_atomic_load(&ctrl->hot_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                        : 0ull;
-                const int ph = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
-                if (ph >= 0 && ph < P) {
-                    const int s = try_claim(ph, 1, got);
+                if (ms.want_open) {
+                    const int s = try_claim(ph, 1, true, got);
                     if (s >= 0) {
                         p_out = ph;
                         s_out = s;
@@ -600,7 +613,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 }
                 const int q = __builtin_amdgcn_readfirstlane(ms.open_young >= 0 ? ms.open_young : ms.open);
                 if (q >= 0) {
-                    const int s = try_claim(q, k, got);
+                    const int s = try_claim(q, k, true, got);
                     if (s >= 0) {
                         // the cursor stays: the next task is likely from the
                         // same problem, whose scene is already staged
@@ -623,7 +636,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 // until a commit reopens one (exponential back-off, up to
                 // 4 x 64 x 64 cycles)
                 const int r = __builtin_amdgcn_readfirstlane(ms.first_run);
-                const int s = r >= 0 ? try_claim(r, k, got) : -1;
+                const int s = r >= 0 ? try_claim(r, k, false, got) : -1;
                 if (s >= 0) {
                     cursor = r;
                     p_out = r;

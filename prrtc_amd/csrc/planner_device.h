@@ -226,6 +226,7 @@ struct MaskScan {
     int first_run;    // first running problem from `from` on, or -1
     int open;         // first open problem from `from` on, or -1
     int open_young;   // first open problem without the aged bit, or -1
+    bool want_open;   // the problem asked about has an open round
 };
 
 struct FirstFrom {   // first set bit at or after a cursor, wrapping around
@@ -247,13 +248,15 @@ struct FirstFrom {   // first set bit at or after a cursor, wrapping around
     }
 };
 
-__device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int words, int from) {
+// `want` (-1: none) is a problem whose open bit is reported in want_open.
+__device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int words, int from, int want) {
     const int lane = lane_id();
     const int w0 = from >> 6;
     const unsigned long long hi_part = ~0ull << (from & 63);
     const unsigned long long *aged = ctrl->aged_mask;
     FirstFrom fr, fo, fy;
     int bits = 0;
+    bool want_open = false;
     for (int base = 0; base < words; base += kWave) {
         const int w = base + lane;
         unsigned long long r = 0ull, o = 0ull, a = 0ull;
@@ -265,6 +268,7 @@ __device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int word
         bits += __builtin_popcountll(r);
         fr.step(r, w, w0, hi_part, base);
         o &= r;
+        want_open |= ballot(want >= 0 && w == (want >> 6) && ((o >> (want & 63)) & 1ull)) != 0;
         fo.step(o, w, w0, hi_part, base);
         if (aged) fy.step(o & ~a, w, w0, hi_part, base);
     }
@@ -275,6 +279,7 @@ __device__ __forceinline__ MaskScan scan_masks(const PlanControl *ctrl, int word
     ms.first_run = fr.get();
     ms.open = fo.get();
     ms.open_young = fy.get();
+    ms.want_open = want_open;
     return ms;
 }
 
