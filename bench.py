@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plans", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-mbm", action="store_true", help="skip the full MBM dataset run")
+    ap.add_argument("--mbm-batch-samples", type=int, default=100000,
+                    help="max_samples of the batched MBM run (tree storage per problem)")
     ap.add_argument("--profile-plan", action="store_true",
                     help="only the C2 Fetch batch (one persistent k_plan launch after one warm-up batch), for "
                          "rocprofv3 runs of the planner kernel")
@@ -529,6 +532,49 @@ def main():
         for p, r, pa in zip(mprobs, mres, mpaths):
             if r["solved"]:
                 paths_to_check.append((p["robot"], p["scene"], pa, 128))
+
+    if not args.skip_mbm:
+        # ----------------------------------------------------- MBM dataset
+        # Every valid problem of the reference's Panda and Fetch problem files
+        # (tests/golden/mbm_*_valid.json.gz) with the reference harness's
+        # settings (scripts/evaluate_mbm.cpp:154-166, max_iters and
+        # max_samples left at 1,000,000): one solve each, as the harness runs
+        # them, then all of them in one batch.  The batch lowers max_samples
+        # to --mbm-batch-samples (tree storage of ~700 problems); that changes
+        # no result unless a tree reaches the limit, which the row-by-row
+        # comparison would show.
+        from prrtc_amd import mbm
+        out["mbm"] = {}
+        for robot in ("panda", "fetch"):
+            data = mbm.load(os.path.join(ROOT, "tests", "golden", f"mbm_{robot}_valid.json.gz"))
+            mst_ = dict(mbm.EVALUATE_SETTINGS)
+            log(f"mbm {robot}: every valid problem, one solve each, then one batch")
+            t0 = time.perf_counter()
+            rows = mbm.evaluate(data, mst_)
+            t_plain = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            brows = mbm.evaluate(data, dict(mst_, max_samples=args.mbm_batch_samples), batch=True)
+            t_batch = time.perf_counter() - t0
+            keys = ("solved", "iters", "cost", "start_tree_size", "goal_tree_size")
+            same = all(all(a[2][k] == b[2][k] for k in keys) and a[3].tobytes() == b[3].tobytes()
+                       for a, b in zip(rows, brows))
+            sm = mbm.summary(rows)
+            n_p = sm["all"]["solved"]
+            n_b = sum(r["solved"] for _, _, r, _ in brows)
+            out["mbm"][robot] = {
+                **sm["all"], "success_rate_per_family": {f: v["success_rate"] for f, v in sm["families"].items()},
+                "max_iters_seen": max(r["iters"] for _, _, r, _ in rows),
+                "plans_per_s_plain": n_p / t_plain, "plans_per_s_batched": n_b / t_batch,
+                "speedup": (n_b / t_batch) / max(1e-9, n_p / t_plain), "plain_wall_s": t_plain,
+                "batch_wall_s": t_batch, "solved_batched": n_b, "identical_results": bool(same),
+                "settings": "the reference harness's (nnc 512, g 32, range 0.5, balance 2, ratio 1, dd on r 4 "
+                            f"alpha 1e-4 min 1, max_iters 1e6); batch max_samples {args.mbm_batch_samples}"}
+            log(f"mbm {robot}: {n_p}/{len(rows)} solved, wall p50 {sm['all']['wall_ms']['p50']:.2f} ms p99 "
+                f"{sm['all']['wall_ms']['p99']:.2f} ms, cost p50 {sm['all']['cost']['p50']:.2f}; batch "
+                f"{out['mbm'][robot]['speedup']:.2f}x, identical={same}")
+            for (fam, idx, r, pa), (_, _, p) in zip(rows, mbm.valid_problems(data)):
+                if r["solved"]:
+                    paths_to_check.append((robot, p["scene"], pa, mst_["granularity"]))
 
     if not args.skip_serving:
         # ---------------------------------------------------------- serving

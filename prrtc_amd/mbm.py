@@ -2,6 +2,7 @@
 the reference's harness (scripts/evaluate_mbm.cpp, scripts/single_mbm.cpp).
 
     python -m prrtc_amd.mbm evaluate <robot> <run_name> --problems FILE [--batch] [--out-dir DIR]
+                                     [--max-iters N] [--max-samples N] [--summary]
     python -m prrtc_amd.mbm single <robot> <problem_name> <problem_idx> --problems FILE
 
 A problem file is the reference's JSON ({"robot", "joints", "problems":
@@ -16,10 +17,15 @@ family, whose cylinders become cuboids with half extents (r, r, r/2)
 ``evaluate`` writes <out-dir>/<robot>_<run_name>.csv with the reference's
 columns (evaluate_mbm.cpp:88-115); with --batch every valid problem of the
 file is solved in one prrtc_solve_batch call instead of one solve each.
+--max-iters and --max-samples replace the harness's defaults (1,000,000
+each; pRRTC_settings.hh:4-5); --summary prints success rate, path cost and
+solve times per family as JSON.  A problem file may be gzip-compressed
+(``.gz``).
 """
 from __future__ import annotations
 
 import argparse
+import gzip
 import json
 import os
 import sys
@@ -41,7 +47,8 @@ CSV_HEADER = ("problem_name,problem_idx,solved,cost,path_length,start_tree_size,
 
 
 def load(path: str) -> dict:
-    with open(path) as f:
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as f:
         return json.load(f)
 
 
@@ -104,6 +111,26 @@ def evaluate(data: dict, settings: dict, out_csv: str | None = None, batch: bool
     return rows
 
 
+def _quantiles(v, qs=(50, 90, 99)):
+    return {f"p{q}": float(np.percentile(v, q)) for q in qs} if len(v) else {}
+
+
+def summary(rows) -> dict:
+    """Success rate, path cost and solve time (API wall and kernel, ms) per
+    family and over all rows of `evaluate`."""
+    def block(rs):
+        solved = [r for r in rs if r["solved"]]
+        return {"problems": len(rs), "solved": len(solved), "success_rate": len(solved) / max(1, len(rs)),
+                "cost": _quantiles([r["cost"] for r in solved]),
+                "wall_ms": _quantiles([r["wall_ns"] / 1e6 for r in rs]),
+                "kernel_ms": _quantiles([r["kernel_ns"] / 1e6 for r in rs]),
+                "iters": _quantiles([r["iters"] for r in solved])}
+    fams = {}
+    for fam, _, r, _ in rows:
+        fams.setdefault(fam, []).append(r)
+    return {"all": block([r for _, _, r, _ in rows]), "families": {f: block(rs) for f, rs in fams.items()}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -113,6 +140,9 @@ def main(argv=None):
     ev.add_argument("--problems", required=True)
     ev.add_argument("--out-dir", default="test_output")
     ev.add_argument("--batch", action="store_true")
+    ev.add_argument("--max-iters", type=int, default=None)
+    ev.add_argument("--max-samples", type=int, default=None)
+    ev.add_argument("--summary", action="store_true")
     sg = sub.add_parser("single")
     sg.add_argument("robot")
     sg.add_argument("problem_name")
@@ -125,9 +155,16 @@ def main(argv=None):
         return 1
     if args.cmd == "evaluate":
         out = os.path.join(args.out_dir, f"{args.robot}_{args.run_name}.csv")
-        rows = evaluate(data, EVALUATE_SETTINGS, out, batch=args.batch)
+        st = dict(EVALUATE_SETTINGS)
+        if args.max_iters is not None:
+            st["max_iters"] = args.max_iters
+        if args.max_samples is not None:
+            st["max_samples"] = args.max_samples
+        rows = evaluate(data, st, out, batch=args.batch)
         failed = sum(1 for _, _, r, _ in rows if not r["solved"])
         print(f"{len(rows)} problems, {failed} failed -> {out}")
+        if args.summary:
+            print(json.dumps(summary(rows)))
         return 0
     from . import lib
     pset = data["problems"][args.problem_name]

@@ -55,3 +55,37 @@ def test_valid_problems_and_csv_format():
     line = mbm.csv_row("cage", 2, res, st)
     assert line.split(", ")[:9] == ["cage", "2", "1", "2.5", "3", "10", "11", "4", "5"]
     assert len(line.split(", ")) == 20
+
+
+def test_full_fixtures_hold_every_valid_problem():
+    """The compressed full-dataset fixtures hold every valid problem of the
+    reference's files (699 Panda, 679 Fetch), with the subset's problems
+    among them unchanged, and their starts pass the oracle's checker (the
+    reference marks them valid)."""
+    from oracle import oracle
+    from prrtc_amd import mbm
+    from prrtc_amd.robots import load_model
+    oracle.build()
+    for robot, n in (("panda", 699), ("fetch", 679)):
+        full = mbm.load(os.path.join(GOLDEN, f"mbm_{robot}_valid.json.gz"))
+        probs = list(mbm.valid_problems(full))
+        assert len(probs) == n
+        sub = subset(robot)
+        for family, pset in sub["problems"].items():
+            by_index = {p["index"]: p for p in full["problems"][family]}
+            for p in pset:
+                assert by_index[p["index"]] == p, (robot, family, p["index"])
+        R = oracle.OracleRobot(load_model(robot))
+        for fam, idx, p in probs[::25]:
+            free, margin = R.check(p["scene"], p["start"][None], margin=True)
+            assert free[0] or margin[0] < 1e-6, (robot, fam, idx)
+
+
+def test_summary_quantiles():
+    from prrtc_amd import mbm
+    rows = [("a", i + 1, {"solved": i % 4 != 0, "cost": float(i), "wall_ns": 1e6 * (i + 1),
+                          "kernel_ns": 5e5 * (i + 1), "iters": i}, None) for i in range(8)]
+    s = mbm.summary(rows)
+    assert s["all"]["problems"] == 8 and s["all"]["solved"] == 6 and s["all"]["success_rate"] == 0.75
+    assert s["all"]["cost"]["p50"] == float(np.percentile([1, 2, 3, 5, 6, 7], 50))
+    assert s["all"]["wall_ms"]["p50"] == 4.5 and s["families"]["a"]["solved"] == 6
