@@ -11,7 +11,12 @@ and measures on the GPU:
   per task x waves per CU; the best sets ``batch_nnc``, ``waves_per_cu`` and
   ``batch_waves_per_task``;
 * plain form (one problem per launch): plans/s over num_new_configs x waves
-  per task on a subset; the best sets ``plain_nnc`` and ``waves_per_task``.
+  per task on a subset; the best sets ``plain_nnc`` and ``waves_per_task``;
+* crowd threshold (the running-problem count above which a workgroup's
+  waves take one slot each): plans/s of the whole problem set under a round
+  budget (--crowd-rounds), where problems that never connect keep long
+  rounds running; a count replaces the default only when it is at least
+  3% faster.  ``--crowd-only`` runs this stage alone on an existing file.
 
 A lane always carries one pose (64 poses per wavefront); the tuned shape is
 how many wavefronts share one task (they split the obstacle tables of each
@@ -112,6 +117,32 @@ def tune_robot(P, robot, n, pilot_iters, max_iters, plain_n):
     return launch, best_b[3], best_p[2], measured
 
 
+CROWD_CANDIDATES = (0, 8, 16, 32)
+
+
+def tune_crowd(P, robot, n, launch, nnc, rounds):
+    """Crowd threshold of a tuned launch shape on every problem of the set
+    (no feasibility filter) under `rounds` rounds: 0 (the default) unless a
+    count is at least 3% faster.  Returns (crowd_above, measurements)."""
+    if launch["batch_waves_per_task"] < 2 or launch["crowd_above"] < 0:
+        return launch["crowd_above"], []
+    probs = problems(robot, n)
+    st = dict(SETTINGS[robot], num_new_configs=nnc, max_iters=rounds)
+    P.set_tuning(robot, launch["waves_per_cu"], launch["waves_per_task"], launch["batch_waves_per_task"], 0)
+    P.solve_batch(probs[:8], dict(st, max_iters=50))
+    rows, rate = [], {}
+    for crowd in CROWD_CANDIDATES:
+        P.set_tuning(robot, launch["waves_per_cu"], launch["waves_per_task"], launch["batch_waves_per_task"], crowd)
+        dt, (res, _, _) = timed(lambda: P.solve_batch(probs, st))
+        rate[crowd] = sum(r["solved"] for r in res) / dt
+        rows.append({"crowd_above": crowd, "problems": len(probs), "max_iters": rounds, "plans_per_s": rate[crowd],
+                     "wall_s": dt})
+        log(f"{robot} crowd {crowd}: {rate[crowd]:.1f} plans/s on {len(probs)} problems, {rounds} rounds at most")
+    P.set_tuning(robot, 0, 0, 0)
+    best = max(rate, key=rate.get)
+    return (best if rate[best] >= 1.03 * rate[0] else 0), rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--robots", default="fetch,panda,baxter")
@@ -120,14 +151,33 @@ def main(argv=None):
     ap.add_argument("--pilot-iters", type=int, default=2000)
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--crowd-rounds", type=int, default=3000)
+    ap.add_argument("--crowd-only", action="store_true",
+                    help="run only the crowd-threshold stage, on the launch shapes of --out")
     args = ap.parse_args(argv)
     import prrtc_amd as P
     P.load()
+    if args.crowd_only:
+        with open(args.out) as f:
+            out = json.load(f)
+        for robot in args.robots.split(","):
+            crowd, rows = tune_crowd(P, robot, args.problems, out["launch"][robot], out["batch_nnc"][robot],
+                                     args.crowd_rounds)
+            out["launch"][robot]["crowd_above"] = crowd
+            out["measured"].setdefault(robot, {})["crowd"] = rows
+            log(f"{robot}: crowd_above {crowd}")
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+        log(f"wrote {args.out}")
+        return
     out = {"launch": {}, "batch_nnc": {}, "plain_nnc": {}, "measured": {},
            "problems": args.problems}
     for robot in args.robots.split(","):
         n = args.problems if robot != "baxter" else max(8, args.problems // 3)
         launch, nb, npl, measured = tune_robot(P, robot, n, args.pilot_iters, args.max_iters, args.plain_problems)
+        # the threshold is a count of running problems: measured on the full
+        # set size for every robot
+        launch["crowd_above"], measured["crowd"] = tune_crowd(P, robot, args.problems, launch, nb, args.crowd_rounds)
         out["launch"][robot] = launch
         out["batch_nnc"][robot] = nb
         out["plain_nnc"][robot] = npl

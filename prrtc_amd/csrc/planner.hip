@@ -130,8 +130,12 @@ struct TaskCtx : CheckCtx {
 // vectors live in LDS: chain[i] holds the chain configurations c_0 = new
 // node, c_{i+1} = c_i + vec; uv holds the nearest node, the steered step and
 // the connect step; red and wbuf are the cross-wave reduction buffers.
+// Inlined into k_plan (its one caller): as a called function it saved and
+// restored ~36 callee-saved VGPRs per task through the stack (144 B per
+// lane), which wrote tens of GB per batch through L2; inlined, the planner
+// kernels use no scratch at all.
 template <class Robot>
-__device__ void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx &ctx_in, float *chain, float *uv,
+__device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx &ctx_in, float *chain, float *uv,
                           float *red, int *wbuf, const Team &tm) {
 #pragma clang fp contract(off)
     // a private copy: its fields stay in registers across the check sequence
