@@ -317,9 +317,10 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     }
 }
 
-// Walk a parent chain from `cur` to its root; write rows when `out` is set.
+// Walk a parent chain from `cur` to its root; write rows (rs floats apart)
+// when `out` is set.
 template <int D>
-__device__ int trace(const PlanDesc &d, int t, int cur, float *out, float *cost, int *root) {
+__device__ int trace(const PlanDesc &d, int t, int cur, float *out, int rs, float *cost, int *root) {
 #pragma clang fp contract(off)
     const float *tn = d.nodes + (size_t)t * D * d.cap;
     const int *par = d.parent + (size_t)t * d.cap;
@@ -332,7 +333,7 @@ __device__ int trace(const PlanDesc &d, int t, int cur, float *out, float *cost,
                 const float a = tn[(size_t)k * d.cap + cur];
                 const float diff = a - tn[(size_t)k * d.cap + p];
                 acc = fmaf(diff, diff, acc);
-                out[(size_t)n * D + k] = a;
+                out[(size_t)n * rs + k] = a;
             }
             *cost += sqrtf(acc);
         }
@@ -442,15 +443,16 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
             st->size[1] = sz[1];
             float cost = 0.0f;
             int root_t, root_o;
-            const int tl = trace<D>(d, win_t, win_last, nullptr, &cost, &root_t);
-            const int ol = trace<D>(d, 1 - win_t, win_nn_o, nullptr, &cost, &root_o);
+            const int rs = ctrl->path_stride > 0 ? ctrl->path_stride : D;
+            const int tl = trace<D>(d, win_t, win_last, nullptr, rs, &cost, &root_t);
+            const int ol = trace<D>(d, 1 - win_t, win_nn_o, nullptr, rs, &cost, &root_o);
             const int off = atomic_add_agent(&ctrl->path_rows, tl + ol);
             if (off + tl + ol > ctrl->path_cap) {
                 st->status = kStatusPathLimit;
             } else {
                 cost = 0.0f;
-                trace<D>(d, win_t, win_last, paths + (size_t)off * D, &cost, &root_t);
-                trace<D>(d, 1 - win_t, win_nn_o, paths + (size_t)(off + tl) * D, &cost, &root_o);
+                trace<D>(d, win_t, win_last, paths + (size_t)off * rs, rs, &cost, &root_t);
+                trace<D>(d, 1 - win_t, win_nn_o, paths + (size_t)(off + tl) * rs, rs, &cost, &root_o);
                 st->t_tree = win_t;
                 st->t_len = tl;
                 st->o_len = ol;
@@ -498,6 +500,53 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
         }
     }
     return status;
+}
+
+// A batch whose problems have different robots runs one persistent grid:
+// AnyRobot stands for all of them.  Its DOF (the largest) lays out the LDS
+// task areas; each task, commit and problem setup dispatches on the
+// problem's robot (a uniform branch), so a problem runs its own robot's code
+// exactly as in a launch of that robot alone, and any free workgroup can
+// take any problem's task.
+struct AnyRobot {
+    static constexpr int DOF = BaxterRobot::DOF;
+};
+static_assert(AnyRobot::DOF >= PandaRobot::DOF && AnyRobot::DOF >= FetchRobot::DOF &&
+                  AnyRobot::DOF >= SphereRobot::DOF,
+              "AnyRobot's DOF bounds every robot's");
+
+// One task in the LDS task area `a` laid out for Robot::DOF (a robot's own
+// layout fits in the larger AnyRobot one).
+template <class Robot>
+__device__ __forceinline__ void run_task(const PlanDesc &d, PlanState *st, int s, const TaskCtx &ctx, float *a,
+                                         int lds_ext, const Team &tm) {
+    constexpr int D = Robot::DOF;
+    float *uv = a + (lds_ext + 1) * D, *red = a + (lds_ext + 5) * D;
+    int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);
+    if constexpr (std::is_same<Robot, AnyRobot>::value) {
+        switch (d.robot) {
+            case PRRTC_ROBOT_SPHERE: plan_task<SphereRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+            case PRRTC_ROBOT_PANDA: plan_task<PandaRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+            case PRRTC_ROBOT_FETCH: plan_task<FetchRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+            case PRRTC_ROBOT_BAXTER: plan_task<BaxterRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+        }
+    } else {
+        plan_task<Robot>(d, st, s, ctx, a, uv, red, wbuf, tm);
+    }
+}
+
+template <class Robot>
+__device__ __forceinline__ void run_commit(const PlanDesc &d, PlanState *st, int p, PlanControl *ctrl, float *paths) {
+    if constexpr (std::is_same<Robot, AnyRobot>::value) {
+        switch (d.robot) {
+            case PRRTC_ROBOT_SPHERE: plan_commit<SphereRobot>(d, st, p, ctrl, paths); break;
+            case PRRTC_ROBOT_PANDA: plan_commit<PandaRobot>(d, st, p, ctrl, paths); break;
+            case PRRTC_ROBOT_FETCH: plan_commit<FetchRobot>(d, st, p, ctrl, paths); break;
+            case PRRTC_ROBOT_BAXTER: plan_commit<BaxterRobot>(d, st, p, ctrl, paths); break;
+        }
+    } else {
+        plan_commit<Robot>(d, st, p, ctrl, paths);
+    }
 }
 
 template <class Robot>
@@ -708,9 +757,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             ctx.obs = crowd ? scene_obs : scene_obs.slice(w, W);
             ctx.waves = crowd ? 1 : W;
             float *a = areas + (crowd ? (size_t)w * area : 0);
-            plan_task<Robot>(d, st, crowd ? s + w : s, ctx, a, a + (lds_ext + 1) * D, a + (lds_ext + 5) * D,
-                             reinterpret_cast<int *>(a + (lds_ext + 5) * D + 2 * kMaxTaskWaves),
-                             crowd ? team_wave() : team_workgroup());
+            run_task<Robot>(d, st, crowd ? s + w : s, ctx, a, lds_ext, crowd ? team_wave() : team_workgroup());
             __threadfence();
         }
         // a shared task is written whole before its slot is counted
@@ -718,7 +765,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
         // the wave that counts the round's last slot commits the round
         if (crowd ? runs : leader) {
             const int prev = wave_atomic_add(&st->done_slots, 1);
-            if (prev == d.nnc - 1) plan_commit<Robot>(d, st, p, ctrl, paths);
+            if (prev == d.nnc - 1) run_commit<Robot>(d, st, p, ctrl, paths);
         }
         if (crowd) __syncthreads();   // ctl read by every wave before the leader claims again
     }
@@ -726,12 +773,8 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
 
 // Per-problem setup: tree roots, Halton states (pRRTC.cu:644-693).
 template <class Robot>
-__global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict__ descs, PlanState *states,
-                                                     const int *__restrict__ list) {
+__device__ __forceinline__ void init_problem(const PlanDesc &d, PlanState *st) {
     constexpr int D = Robot::DOF;
-    const int e = list ? list[blockIdx.x] : (int)blockIdx.x;   // problem (table entry) to set up
-    const PlanDesc &d = descs[e];
-    PlanState *st = states + e;
     const int lane = lane_id();
     for (int i = lane; i < 1 + d.n_goals; i += kWave) {
         const int t = i == 0 ? 0 : 1, idx = i == 0 ? 0 : i - 1;
@@ -753,6 +796,24 @@ __global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict_
         st->size[0] = 1;
         st->size[1] = d.n_goals;
         st->next_slot = 0;
+    }
+}
+
+template <class Robot>
+__global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict__ descs, PlanState *states,
+                                                     const int *__restrict__ list) {
+    const int e = list ? list[blockIdx.x] : (int)blockIdx.x;   // problem (table entry) to set up
+    const PlanDesc &d = descs[e];
+    PlanState *st = states + e;
+    if constexpr (std::is_same<Robot, AnyRobot>::value) {
+        switch (d.robot) {
+            case PRRTC_ROBOT_SPHERE: init_problem<SphereRobot>(d, st); break;
+            case PRRTC_ROBOT_PANDA: init_problem<PandaRobot>(d, st); break;
+            case PRRTC_ROBOT_FETCH: init_problem<FetchRobot>(d, st); break;
+            case PRRTC_ROBOT_BAXTER: init_problem<BaxterRobot>(d, st); break;
+        }
+    } else {
+        init_problem<Robot>(d, st);
     }
 }
 
@@ -841,7 +902,7 @@ struct Workspace {
 };
 
 static Workspace &workspace(int robot) {
-    static thread_local Workspace *ws[4] = {};   // released with the process
+    static thread_local Workspace *ws[kRobotMixed + 1] = {};   // released with the process
     if (!ws[robot]) ws[robot] = new Workspace();
     ws[robot]->ensure_stream();
     return *ws[robot];
@@ -937,6 +998,7 @@ void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *d
     d.balance = hp.st->balance;
     d.dd = hp.st->dynamic_domain ? 1 : 0;
     d.n_goals = hp.n_goals;
+    d.robot = hp.robot;
     d.cap = L.cap;
     d.max_ext = L.max_ext;
     d.scene_f4 = (L.scene_floats - 8) / 4;
@@ -957,22 +1019,24 @@ void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *d
     d.seed = hp.st->seed;
 }
 
-int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int path_cap, float *out) {
+int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int rs, int path_cap, float *out) {
     const int D = robot_dof(hp.robot);
     const int n_rows = ps.t_len + ps.o_len + 2;
     if (n_rows > path_cap) return -1;
-    const float *tr = rows + (size_t)ps.path_off * D;      // t-trace then o-trace
-    const float *p1 = ps.t_tree == 1 ? tr : tr + (size_t)ps.t_len * D;
-    const float *p0 = ps.t_tree == 1 ? tr + (size_t)ps.t_len * D : tr;
+    const float *tr = rows + (size_t)ps.path_off * rs;      // t-trace then o-trace
+    const float *p1 = ps.t_tree == 1 ? tr : tr + (size_t)ps.t_len * rs;
+    const float *p0 = ps.t_tree == 1 ? tr + (size_t)ps.t_len * rs : tr;
     const int n1 = ps.t_tree == 1 ? ps.t_len : ps.o_len;
     const int n0 = ps.t_tree == 1 ? ps.o_len : ps.t_len;
     int row = 0;
     std::memcpy(out + (size_t)row++ * D, hp.goals + (size_t)ps.reached * D, sizeof(float) * D);
-    for (int k = n1 - 1; k >= 0; k--) std::memcpy(out + (size_t)row++ * D, p1 + (size_t)k * D, sizeof(float) * D);
-    for (int k = 0; k < n0; k++) std::memcpy(out + (size_t)row++ * D, p0 + (size_t)k * D, sizeof(float) * D);
+    for (int k = n1 - 1; k >= 0; k--) std::memcpy(out + (size_t)row++ * D, p1 + (size_t)k * rs, sizeof(float) * D);
+    for (int k = 0; k < n0; k++) std::memcpy(out + (size_t)row++ * D, p0 + (size_t)k * rs, sizeof(float) * D);
     std::memcpy(out + (size_t)row++ * D, hp.start, sizeof(float) * D);
     return row;
 }
+
+int plan_dof(int robot) { return robot == kRobotMixed ? AnyRobot::DOF : robot_dof(robot); }
 
 void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, const int *d_list, int n,
                       hipStream_t s) {
@@ -984,6 +1048,7 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
         PRRTC_INIT_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
         PRRTC_INIT_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
         PRRTC_INIT_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+        PRRTC_INIT_CASE(kRobotMixed, AnyRobot)
 #undef PRRTC_INIT_CASE
     }
     PRRTC_HIP(hipGetLastError());
@@ -998,7 +1063,7 @@ static size_t plan_lds_bytes(int D, int lds_scene_f4, int max_ext, int waves) {
 
 void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, PlanControl *d_ctrl, float *d_paths,
                        int grid, int waves, int lds_scene_f4, int max_ext, hipStream_t s) {
-    const size_t lds = plan_lds_bytes(robot_dof(robot), lds_scene_f4, max_ext, waves);
+    const size_t lds = plan_lds_bytes(plan_dof(robot), lds_scene_f4, max_ext, waves);
     switch (robot) {
 #define PRRTC_SLICE_CASE(ID, R)                                                                              \
     case ID:                                                                                                 \
@@ -1009,6 +1074,7 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
         PRRTC_SLICE_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
         PRRTC_SLICE_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
         PRRTC_SLICE_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+        PRRTC_SLICE_CASE(kRobotMixed, AnyRobot)
 #undef PRRTC_SLICE_CASE
     }
     PRRTC_HIP(hipGetLastError());
@@ -1130,6 +1196,7 @@ int plan_grid(int robot, int waves, long long slots) {
 // A robot group in flight: its problems and where its state lives.
 struct GroupRun {
     std::vector<HostProblem> probs;
+    int robot = -1;              // the problems' robot, or kRobotMixed
     bool graph_rounds = false;   // graph-replayed round grids instead of one persistent grid
     Workspace *ws = nullptr;
     char *dev = nullptr, *host = nullptr;
@@ -1141,9 +1208,19 @@ struct GroupRun {
 static void start_group(GroupRun &gr, int path_cap) {
     const std::vector<HostProblem> &probs = gr.probs;
     const int P = (int)probs.size();
-    const int robot = probs[0].robot;
-    const int D = robot_dof(robot);
+    const int robot = gr.robot;
+    const int D = plan_dof(robot);
     Workspace &ws = workspace(robot);
+    // a mixed launch takes, of each launch shape, the largest value among
+    // its problems' robots
+    std::vector<int> robots;
+    for (const HostProblem &hp : probs)
+        if (std::find(robots.begin(), robots.end(), hp.robot) == robots.end()) robots.push_back(hp.robot);
+    auto largest = [&](auto f) {
+        int v = 0;
+        for (int r : robots) v = std::max(v, f(r));
+        return v;
+    };
     gr.ws = &ws;
     // --- layout
     Align dl;
@@ -1189,9 +1266,10 @@ static void start_group(GroupRun &gr, int path_cap) {
     unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + h_hdr + o_mask);
     for (int w = 0; w < words; w++) hm[w] = hm[words + w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
+    hc->path_stride = D;
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     hc->prio_policy = kPrioOldest;
-    hc->crowd_above = P > 1 ? plan_crowd_above(robot, hc->max_nnc) : 0;
+    hc->crowd_above = P > 1 ? largest([&](int r) { return plan_crowd_above(r, hc->max_nnc); }) : 0;
     // --- upload (descs + control in one copy; scenes and init rows per
     // problem): the control block is complete before the copy is queued
     // (the copy reads the pinned staging asynchronously)
@@ -1204,7 +1282,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     // persistent grid either way
     long long total_slots = 0;
     for (int i = 0; i < P; i++) total_slots += probs[i].st->num_new_configs;
-    const int waves = P == 1 ? plan_task_waves(robot) : plan_batch_waves(robot);
+    const int waves = P == 1 ? plan_task_waves(robot) : largest(plan_batch_waves);
     const PlanDesc *d_desc = reinterpret_cast<const PlanDesc *>(dev + o_desc);
     PlanState *d_state = reinterpret_cast<PlanState *>(dev + o_state);
     PlanControl *d_ctrl = reinterpret_cast<PlanControl *>(dev + o_ctrl);
@@ -1217,7 +1295,8 @@ static void start_group(GroupRun &gr, int path_cap) {
         run_round_graph(robot, d_desc, d_state, d_ctrl, d_paths, P, hc->max_nnc, waves, lds_f4, ext_all,
                         hc->run_mask, reinterpret_cast<unsigned long long *>(host + h_run), words, max_rounds, s);
     } else {
-        launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, plan_grid(robot, waves, total_slots), waves,
+        const int grid = largest([&](int r) { return plan_grid(r, waves, total_slots); });
+        launch_plan_slice(robot, d_desc, d_state, d_ctrl, d_paths, grid, waves,
                           lds_f4, ext_all, s);
     }
     PRRTC_HIP(hipEventRecord(ws.e1, s));
@@ -1238,7 +1317,7 @@ static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int 
                          int64_t *kernel_ns, int *rounds) {
     const std::vector<HostProblem> &probs = gr.probs;
     const int P = (int)probs.size();
-    const int D = robot_dof(probs[0].robot);
+    const int D = plan_dof(gr.robot);   // floats per path row (the launch's path_stride)
     Workspace &ws = *gr.ws;
     const hipStream_t s = ws.stream;
     PRRTC_HIP(hipStreamSynchronize(s));
@@ -1271,7 +1350,7 @@ static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int 
         r.path_nodes = 0;
         r.path_length = 0;
         if (!r.solved) continue;
-        const int row = assemble_path(hp, ps, rows.data(), path_cap, paths + (size_t)hp.index * path_stride);
+        const int row = assemble_path(hp, ps, rows.data(), D, path_cap, paths + (size_t)hp.index * path_stride);
         if (row < 0) {
             r.solved = 0;
             r.status = PRRTC_EOVERFLOW;
@@ -1304,6 +1383,14 @@ int check_problem(const HostProblem &hp, const char *fn) {
     return PRRTC_OK;
 }
 
+static bool mixed_grid() {
+    static const bool on = [] {
+        const char *e = std::getenv("PRRTC_MIXED_GRID");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, float *paths, int path_cap,
                       int path_stride, prrtc_batch_stats *stats, const char *fn, bool graph_rounds = false) {
     const int64_t t0 = now_ns();
@@ -1317,14 +1404,21 @@ static int solve_many(std::vector<HostProblem> &probs, prrtc_result *results, fl
     }
     std::map<int, std::vector<HostProblem>> groups;
     for (auto &hp : probs) groups[hp.robot].push_back(hp);
+    // Problems of several robots share one persistent grid (kRobotMixed):
+    // any workgroup takes any problem's task, so one robot's long tail runs
+    // beside another's bulk.  The graph-replayed comparator, and
+    // PRRTC_MIXED_GRID=0, keep one grid per robot, each on its own stream.
+    if (groups.size() > 1 && !graph_rounds && mixed_grid()) {
+        groups.clear();
+        groups[kRobotMixed] = probs;
+    }
     int64_t kernel_ns = 0;
     int rounds = 0;
-    // every robot's group runs its persistent grid on its own stream; the
-    // groups overlap on the device
     std::vector<GroupRun> runs(groups.size());
     size_t k = 0;
     for (auto &kv : groups) {
         runs[k].probs = std::move(kv.second);
+        runs[k].robot = kv.first;
         runs[k].graph_rounds = graph_rounds;
         start_group(runs[k++], path_cap);
     }

@@ -35,7 +35,9 @@ constexpr int kMaxNnc = 1024;
 struct PlanDesc {
     int nnc, g, max_iters, max_samples;
     int balance, dd, n_goals, cap;
-    int max_ext, scene_f4, pad0, pad1;
+    int max_ext, scene_f4;
+    int robot;            // PRRTC_ROBOT_*: the planner code a mixed launch dispatches to
+    int pad1;
     float range, tree_ratio, dd_alpha, dd_radius;
     float dd_min_radius, pad2, pad3, pad4;
     const float *scene;   // packed scene
@@ -76,7 +78,10 @@ struct PlanControl {
     int max_nnc;               // largest num_new_configs of the launch
     int mask_words;            // ceil(n_problems / 64)
     int prio_policy;           // wave priority of a task: kPrioNone / kPrioOldest / kPrioYoungest
-    int pad0, pad1;
+    // floats between path rows in the path output: the largest DOF of a
+    // launch whose problems have different robots (0: the robot's DOF)
+    int path_stride;
+    int pad1;
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
     unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit

@@ -29,6 +29,13 @@ struct BlockLayout {
     int cap = 0, max_ext = 0, scene_floats = 0;
 };
 
+// Internal robot id of a launch whose problems have different robots (one
+// persistent grid for all of them; never seen by callers of the C ABI).
+constexpr int kRobotMixed = 4;
+// Floats per configuration of a launch: the robot's DOF, or the largest DOF
+// for kRobotMixed (the layout of its task areas and path rows).
+int plan_dof(int robot);
+
 // Rejects what the planner cannot run (unknown robot, NaN, oversized scene,
 // bad settings, chains over kMaxExt); sets the thread's error message.
 int check_problem(const HostProblem &hp, const char *fn);
@@ -80,8 +87,9 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
                        int grid, int waves, int lds_scene_f4, int max_ext, hipStream_t s);
 
 // Rows of one solution: goal -> start (pRRTC.cu:715-746) from the two traces
-// the commit wrote at state.path_off.  Returns the row count, or -1 if the
-// path does not fit path_cap rows.
-int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int path_cap, float *out);
+// the commit wrote at state.path_off, `rs` floats per row in `rows` (the
+// launch's PlanControl::path_stride); `out` gets the robot's DOF per row.
+// Returns the row count, or -1 if the path does not fit path_cap rows.
+int assemble_path(const HostProblem &hp, const PlanState &ps, const float *rows, int rs, int path_cap, float *out);
 
 }  // namespace prrtc

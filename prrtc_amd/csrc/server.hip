@@ -441,7 +441,7 @@ void ServerImpl::harvest(RobotClass &c, std::vector<std::pair<uint64_t, Finished
         r.cost = r.solved ? ps.cost : 0.0f;
         if (r.solved) {
             f.path.resize((size_t)cfg.path_cap * c.D);
-            const int n = assemble_path(rq->hp(), ps, rows.data(), cfg.path_cap, f.path.data());
+            const int n = assemble_path(rq->hp(), ps, rows.data(), c.D, cfg.path_cap, f.path.data());
             if (n < 0) {
                 r.solved = 0;
                 r.status = PRRTC_EOVERFLOW;

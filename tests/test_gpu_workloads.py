@@ -90,8 +90,8 @@ def test_c2_subset_batched(gpu, oracle_robots):
 
 
 def test_c3_scaled_mix(gpu, oracle_robots):
-    """A 4 Baxter + 8 Panda + 4 Fetch shelf mix in one call (the three robot
-    grids overlap on the device), the C3 settings: paths re-validated, each
+    """A 4 Baxter + 8 Panda + 4 Fetch shelf mix in one call (one persistent
+    grid serves the three robots), the C3 settings: paths re-validated, each
     result bitwise the one alone."""
     from prrtc_amd import workloads
     probs = workloads.c3_problems(4, 8, 4, seed=0)
