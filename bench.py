@@ -21,7 +21,11 @@ each step is one launch of the edge-validation kernel over all edges.
     batched_per_robot  the same comparison for Panda and Baxter on the
                  shelf problem sets of the mix (each class's settings)
     mix          128 Baxter + 256 Panda + 128 Fetch shelf problems (g 128,
-                 dynamic domain, balance 1) in one call
+                 dynamic domain, balance 1) in one call: one persistent grid
+                 serves the three robots
+    mbm          every valid problem of the reference's Panda and Fetch MBM
+                 problem files with its harness's settings: success rate,
+                 cost, solve times; one solve each vs one batch
     serving      Poisson request stream (Panda:Fetch:Baxter 2:1:1, fresh
                  4-128 primitive scenes) through the continuous-batching
                  server: p50/p99 latency, admission wait, plans/s
@@ -65,15 +69,6 @@ def time_launches(fn, steps, torch):
 
 def pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
-
-
-def load_tuned():
-    """Launch shapes and num_new_configs picked by prrtc_amd.autotune."""
-    path = os.path.join(ROOT, "prrtc_amd", "tuned.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f)
-    return {}
 
 
 def main():
@@ -129,10 +124,9 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream().cuda_stream
     g = args.granularity
-    tuned = load_tuned()
-    for robot, t in tuned.get("launch", {}).items():
-        P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0),
-                     t.get("crowd_above", 0))
+    # launch shapes and num_new_configs picked by prrtc_amd.autotune
+    from prrtc_amd import autotune
+    tuned = autotune.apply()
 
     if args.profile_plan:
         bprobs = workloads.c2_problems(args.batch, seed=0)
@@ -525,8 +519,7 @@ def main():
                       "solved_plain": n_mp, "plans_per_s_plain": n_mp / t_mix_plain, "plain_wall_s": t_mix_plain,
                       "speedup": (n_m / t_mix) / max(1e-9, n_mp / t_mix_plain), "identical_results": bool(m_same),
                       "settings": "nnc 128, g 128, balance 1, ratio 0.5, dd on, range 0.5, max_iters 3000; one persistent "
-                                  "grid per robot class, all classes concurrently; plain: the same settings, one "
-                                  "problem per launch"}
+                                  "grid for the three robots; plain: the same settings, one problem per launch"}
         log(f"mix: {n_m}/{len(mprobs)} solved in {t_mix:.3f} s ({n_m / t_mix:.1f} plans/s) vs plain "
             f"{n_mp / t_mix_plain:.1f} plans/s ({out['mix']['speedup']:.2f}x), identical={m_same}")
         for p, r, pa in zip(mprobs, mres, mpaths):

@@ -55,6 +55,20 @@ SETTINGS = {
 }
 
 
+def apply(path: str = DEFAULT_OUT) -> dict:
+    """Set every robot's launch shapes from a tuned file (prrtc_set_tuning);
+    returns the file's contents, {} when there is no file."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        tuned = json.load(f)
+    import prrtc_amd as P
+    for robot, t in tuned.get("launch", {}).items():
+        P.set_tuning(robot, t.get("waves_per_cu", 0), t.get("waves_per_task", 0), t.get("batch_waves_per_task", 0),
+                     t.get("crowd_above", 0))
+    return tuned
+
+
 def log(msg):
     print(f"[autotune] {msg}", file=sys.stderr, flush=True)
 

@@ -110,9 +110,12 @@ def main(argv=None):
     ap.add_argument("--young-rounds", type=int, default=0)
     ap.add_argument("--deadline-ms", type=int, default=0, help="per-request deadline (0: none)")
     ap.add_argument("--out", default=None, help="write the summaries as JSON")
+    ap.add_argument("--tuned", default=None, help="tuned launch shapes to apply (default: prrtc_amd/tuned.json)")
     args = ap.parse_args(argv)
     import prrtc_amd as P
+    from prrtc_amd import autotune
     P.load()
+    autotune.apply(args.tuned or autotune.DEFAULT_OUT)
     print(f"[loadgen] building {args.pool} requests", file=sys.stderr, flush=True)
     pool = request_pool(args.pool)
     rows = []
