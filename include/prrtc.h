@@ -33,6 +33,8 @@ extern "C" {
 #define PRRTC_ROBOT_PANDA 1  /* 7 DoF  (Robots.hh:10-64)   */
 #define PRRTC_ROBOT_FETCH 2  /* 8 DoF  (Robots.hh:66-120)  */
 #define PRRTC_ROBOT_BAXTER 3 /* 14 DoF (Robots.hh:122-188) */
+#define PRRTC_ROBOT_UR5 4    /* 6 DoF: not in the reference; compiled from prrtc_amd/robots/ur5.urdf +
+                                ur5_spheres.json by the robot-model compiler */
 
 /* Obstacle tables, row-major float32.  The five tables the reference's
  * checker reads (src/planning/utils.cuh:211-242; shapes at

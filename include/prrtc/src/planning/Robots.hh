@@ -51,6 +51,17 @@ struct Baxter {
     PRRTC_ROBOT_COMMON(14)
 };
 
+// UR5 (not in the reference): the limits of its model table, compiled from
+// prrtc_amd/robots/ur5.urdf and ur5_spheres.json by the robot-model compiler.
+struct UR5 {
+    static constexpr auto name = "ur5";
+    static constexpr int id = PRRTC_ROBOT_UR5;
+    static constexpr float s_m[6] = {6.2831853f, 3.14159265f, 6.2831853f, 6.2831853f, 6.2831853f, 6.2831853f};
+    static constexpr float s_a[6] = {-3.14159265f, -3.14159265f, -3.14159265f, -3.14159265f, -3.14159265f,
+                                     -3.14159265f};
+    PRRTC_ROBOT_COMMON(6)
+};
+
 // The 3-DoF point robot: one sphere of radius 1 at q.
 struct Sphere {
     static constexpr auto name = "sphere";

@@ -14,6 +14,7 @@
 #include "robots/baxter.gen.h"
 #include "robots/fetch.gen.h"
 #include "robots/panda.gen.h"
+#include "robots/ur5.gen.h"
 
 namespace prrtc {
 
@@ -382,6 +383,18 @@ struct SphereRobot {     // Robots.hh:190-237; fkcc<Sphere>, utils.cuh:330-334
 PRRTC_GEN_ROBOT(Panda, panda)
 PRRTC_GEN_ROBOT(Fetch, fetch)
 PRRTC_GEN_ROBOT(Baxter, baxter)
+PRRTC_GEN_ROBOT(Ur5, ur5)
 #undef PRRTC_GEN_ROBOT
+
+// Every robot the library is built for, as X(id, struct): the one list the
+// dispatch switches expand.  A robot from the model compiler is one more
+// line here, its generated header above and its id in include/prrtc.h.
+#define PRRTC_FOR_EACH_ROBOT(X)        \
+    X(PRRTC_ROBOT_SPHERE, SphereRobot) \
+    X(PRRTC_ROBOT_PANDA, PandaRobot)   \
+    X(PRRTC_ROBOT_FETCH, FetchRobot)   \
+    X(PRRTC_ROBOT_BAXTER, BaxterRobot) \
+    X(PRRTC_ROBOT_UR5, Ur5Robot)
+constexpr int kNumRobots = 5;   // ids 0 .. kNumRobots - 1
 
 }  // namespace prrtc

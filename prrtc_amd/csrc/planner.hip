@@ -508,12 +508,16 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
 // problem's robot (a uniform branch), so a problem runs its own robot's code
 // exactly as in a launch of that robot alone, and any free workgroup can
 // take any problem's task.
+constexpr int max_robot_dof() {
+    int m = 0;
+#define PRRTC_DOF_MAX(ID, R) m = R::DOF > m ? R::DOF : m;
+    PRRTC_FOR_EACH_ROBOT(PRRTC_DOF_MAX)
+#undef PRRTC_DOF_MAX
+    return m;
+}
 struct AnyRobot {
-    static constexpr int DOF = BaxterRobot::DOF;
+    static constexpr int DOF = max_robot_dof();
 };
-static_assert(AnyRobot::DOF >= PandaRobot::DOF && AnyRobot::DOF >= FetchRobot::DOF &&
-                  AnyRobot::DOF >= SphereRobot::DOF,
-              "AnyRobot's DOF bounds every robot's");
 
 // One task in the LDS task area `a` laid out for Robot::DOF (a robot's own
 // layout fits in the larger AnyRobot one).
@@ -525,10 +529,9 @@ __device__ __forceinline__ void run_task(const PlanDesc &d, PlanState *st, int s
     int *wbuf = reinterpret_cast<int *>(red + 2 * kMaxTaskWaves);
     if constexpr (std::is_same<Robot, AnyRobot>::value) {
         switch (d.robot) {
-            case PRRTC_ROBOT_SPHERE: plan_task<SphereRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
-            case PRRTC_ROBOT_PANDA: plan_task<PandaRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
-            case PRRTC_ROBOT_FETCH: plan_task<FetchRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
-            case PRRTC_ROBOT_BAXTER: plan_task<BaxterRobot>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+#define PRRTC_TASK_CASE(ID, R) case ID: plan_task<R>(d, st, s, ctx, a, uv, red, wbuf, tm); break;
+            PRRTC_FOR_EACH_ROBOT(PRRTC_TASK_CASE)
+#undef PRRTC_TASK_CASE
         }
     } else {
         plan_task<Robot>(d, st, s, ctx, a, uv, red, wbuf, tm);
@@ -539,10 +542,9 @@ template <class Robot>
 __device__ __forceinline__ void run_commit(const PlanDesc &d, PlanState *st, int p, PlanControl *ctrl, float *paths) {
     if constexpr (std::is_same<Robot, AnyRobot>::value) {
         switch (d.robot) {
-            case PRRTC_ROBOT_SPHERE: plan_commit<SphereRobot>(d, st, p, ctrl, paths); break;
-            case PRRTC_ROBOT_PANDA: plan_commit<PandaRobot>(d, st, p, ctrl, paths); break;
-            case PRRTC_ROBOT_FETCH: plan_commit<FetchRobot>(d, st, p, ctrl, paths); break;
-            case PRRTC_ROBOT_BAXTER: plan_commit<BaxterRobot>(d, st, p, ctrl, paths); break;
+#define PRRTC_COMMIT_CASE(ID, R) case ID: plan_commit<R>(d, st, p, ctrl, paths); break;
+            PRRTC_FOR_EACH_ROBOT(PRRTC_COMMIT_CASE)
+#undef PRRTC_COMMIT_CASE
         }
     } else {
         plan_commit<Robot>(d, st, p, ctrl, paths);
@@ -807,10 +809,9 @@ __global__ __launch_bounds__(kWave) void k_plan_init(const PlanDesc *__restrict_
     PlanState *st = states + e;
     if constexpr (std::is_same<Robot, AnyRobot>::value) {
         switch (d.robot) {
-            case PRRTC_ROBOT_SPHERE: init_problem<SphereRobot>(d, st); break;
-            case PRRTC_ROBOT_PANDA: init_problem<PandaRobot>(d, st); break;
-            case PRRTC_ROBOT_FETCH: init_problem<FetchRobot>(d, st); break;
-            case PRRTC_ROBOT_BAXTER: init_problem<BaxterRobot>(d, st); break;
+#define PRRTC_SETUP_CASE(ID, R) case ID: init_problem<R>(d, st); break;
+            PRRTC_FOR_EACH_ROBOT(PRRTC_SETUP_CASE)
+#undef PRRTC_SETUP_CASE
         }
     } else {
         init_problem<Robot>(d, st);
@@ -925,10 +926,7 @@ int robot_limits(int robot, float *lo, float *span) {
     switch (robot) {
 #define PRRTC_LIM(ID, R) \
     case ID: for (int i = 0; i < R::DOF; i++) { lo[i] = R::lower(i); span[i] = R::span(i); } return R::DOF;
-        PRRTC_LIM(PRRTC_ROBOT_SPHERE, SphereRobot)
-        PRRTC_LIM(PRRTC_ROBOT_PANDA, PandaRobot)
-        PRRTC_LIM(PRRTC_ROBOT_FETCH, FetchRobot)
-        PRRTC_LIM(PRRTC_ROBOT_BAXTER, BaxterRobot)
+        PRRTC_FOR_EACH_ROBOT(PRRTC_LIM)
 #undef PRRTC_LIM
         default: return -1;
     }
@@ -1044,10 +1042,7 @@ void launch_plan_init(int robot, const PlanDesc *d_desc, PlanState *d_state, con
     switch (robot) {
 #define PRRTC_INIT_CASE(ID, R) \
     case ID: hipLaunchKernelGGL(k_plan_init<R>, dim3(n), dim3(kWave), 0, s, d_desc, d_state, d_list); break;
-        PRRTC_INIT_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
-        PRRTC_INIT_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
-        PRRTC_INIT_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
-        PRRTC_INIT_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+        PRRTC_FOR_EACH_ROBOT(PRRTC_INIT_CASE)
         PRRTC_INIT_CASE(kRobotMixed, AnyRobot)
 #undef PRRTC_INIT_CASE
     }
@@ -1070,10 +1065,7 @@ void launch_plan_slice(int robot, const PlanDesc *d_desc, PlanState *d_state, Pl
         hipLaunchKernelGGL(k_plan<R>, dim3(grid), dim3(waves * kWave), lds, s, d_desc, d_state, d_ctrl,      \
                            d_paths, lds_scene_f4, max_ext);                                                  \
         break;
-        PRRTC_SLICE_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
-        PRRTC_SLICE_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
-        PRRTC_SLICE_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
-        PRRTC_SLICE_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+        PRRTC_FOR_EACH_ROBOT(PRRTC_SLICE_CASE)
         PRRTC_SLICE_CASE(kRobotMixed, AnyRobot)
 #undef PRRTC_SLICE_CASE
     }
@@ -1103,10 +1095,7 @@ void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, Plan
                            max_nnc, lds_scene_f4, max_ext);                                                      \
         hipLaunchKernelGGL(k_round_commit<R>, dim3(P), dim3(kWave), 0, s, d_desc, d_state, d_ctrl, d_paths);     \
         break;
-            PRRTC_ROUND_CASE(PRRTC_ROBOT_SPHERE, SphereRobot)
-            PRRTC_ROUND_CASE(PRRTC_ROBOT_PANDA, PandaRobot)
-            PRRTC_ROUND_CASE(PRRTC_ROBOT_FETCH, FetchRobot)
-            PRRTC_ROUND_CASE(PRRTC_ROBOT_BAXTER, BaxterRobot)
+            PRRTC_FOR_EACH_ROBOT(PRRTC_ROUND_CASE)
 #undef PRRTC_ROUND_CASE
         }
     }
@@ -1132,10 +1121,10 @@ void run_round_graph(int robot, const PlanDesc *d_desc, PlanState *d_state, Plan
 }
 
 // Launch shape of each robot's persistent grids (prrtc_set_tuning; 0 = default).
-static std::atomic<int> g_waves_per_cu[4] = {{0}, {0}, {0}, {0}};
-static std::atomic<int> g_task_waves[4] = {{0}, {0}, {0}, {0}};
-static std::atomic<int> g_batch_waves[4] = {{0}, {0}, {0}, {0}};
-static std::atomic<int> g_crowd_above[4] = {{0}, {0}, {0}, {0}};
+static std::atomic<int> g_waves_per_cu[kNumRobots] = {};
+static std::atomic<int> g_task_waves[kNumRobots] = {};
+static std::atomic<int> g_batch_waves[kNumRobots] = {};
+static std::atomic<int> g_crowd_above[kNumRobots] = {};
 
 static int env_int(const char *name, int lo, int hi) {
     const char *e = std::getenv(name);
@@ -1144,7 +1133,7 @@ static int env_int(const char *name, int lo, int hi) {
 }
 
 static int tuned(const std::atomic<int> *table, int robot, int env, int dflt) {
-    const int v = robot >= 0 && robot < 4 ? table[robot].load(std::memory_order_relaxed) : 0;
+    const int v = robot >= 0 && robot < kNumRobots ? table[robot].load(std::memory_order_relaxed) : 0;
     return v > 0 ? v : (env > 0 ? env : dflt);
 }
 
@@ -1175,7 +1164,7 @@ static int crowd_setting(int robot) {
         const char *e = std::getenv("PRRTC_CROWD_ABOVE");
         return e ? std::atoi(e) : 0;
     }();
-    const int v = robot >= 0 && robot < 4 ? g_crowd_above[robot].load(std::memory_order_relaxed) : 0;
+    const int v = robot >= 0 && robot < kNumRobots ? g_crowd_above[robot].load(std::memory_order_relaxed) : 0;
     return v != 0 ? v : env;
 }
 
@@ -1463,12 +1452,12 @@ int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, 
 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
     auto bad = [](int v, int hi) { return v < 0 || v > hi; };
-    if (!t || robot < -1 || robot > 3 || bad(t->waves_per_cu, kMaxWavesPerCu) ||
+    if (!t || robot < -1 || robot >= kNumRobots || bad(t->waves_per_cu, kMaxWavesPerCu) ||
         bad(t->waves_per_task, kMaxTaskWaves) || bad(t->batch_waves_per_task, kMaxTaskWaves)) {
         set_error("prrtc_set_tuning: invalid argument");
         return PRRTC_EINVAL;
     }
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < kNumRobots; r++) {
         if (robot >= 0 && r != robot) continue;
         g_waves_per_cu[r].store(t->waves_per_cu, std::memory_order_relaxed);
         g_task_waves[r].store(t->waves_per_task, std::memory_order_relaxed);
@@ -1479,7 +1468,7 @@ int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
 }
 
 int prrtc_get_tuning(int robot, prrtc_tuning *t) {
-    if (!t || robot < 0 || robot > 3) {
+    if (!t || robot < 0 || robot >= kNumRobots) {
         set_error("prrtc_get_tuning: invalid argument");
         return PRRTC_EINVAL;
     }

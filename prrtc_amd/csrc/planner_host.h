@@ -31,7 +31,7 @@ struct BlockLayout {
 
 // Internal robot id of a launch whose problems have different robots (one
 // persistent grid for all of them; never seen by callers of the C ABI).
-constexpr int kRobotMixed = 4;
+constexpr int kRobotMixed = kNumRobots;
 // Floats per configuration of a launch: the robot's DOF, or the largest DOF
 // for kRobotMixed (the layout of its task areas and path rows).
 int plan_dof(int robot);

@@ -33,7 +33,7 @@
 namespace prrtc {
 namespace {
 
-constexpr int kRobots = 4;
+constexpr int kRobots = kNumRobots;
 
 // First-fit sub-allocator over [0, total); sizes are multiples of 256.
 class Arena {

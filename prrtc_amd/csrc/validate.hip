@@ -121,20 +121,18 @@ void set_error(const char *fmt, ...) {
 
 int robot_dof(int robot) {
     switch (robot) {
-        case PRRTC_ROBOT_SPHERE: return SphereRobot::DOF;
-        case PRRTC_ROBOT_PANDA: return PandaRobot::DOF;
-        case PRRTC_ROBOT_FETCH: return FetchRobot::DOF;
-        case PRRTC_ROBOT_BAXTER: return BaxterRobot::DOF;
+#define PRRTC_DOF_CASE(ID, R) case ID: return R::DOF;
+        PRRTC_FOR_EACH_ROBOT(PRRTC_DOF_CASE)
+#undef PRRTC_DOF_CASE
         default: return -1;
     }
 }
 
 int robot_num_spheres(int robot) {
     switch (robot) {
-        case PRRTC_ROBOT_SPHERE: return SphereRobot::NSPHERES;
-        case PRRTC_ROBOT_PANDA: return PandaRobot::NSPHERES;
-        case PRRTC_ROBOT_FETCH: return FetchRobot::NSPHERES;
-        case PRRTC_ROBOT_BAXTER: return BaxterRobot::NSPHERES;
+#define PRRTC_NSPH_CASE(ID, R) case ID: return R::NSPHERES;
+        PRRTC_FOR_EACH_ROBOT(PRRTC_NSPH_CASE)
+#undef PRRTC_NSPH_CASE
         default: return -1;
     }
 }
@@ -241,12 +239,15 @@ static void launch_fk(const float *d_poses, int n, float *d_out, hipStream_t st)
     PRRTC_HIP(hipGetLastError());
 }
 
-#define PRRTC_DISPATCH(robot, FN, ...)                                   \
-    switch (robot) {                                                     \
-        case PRRTC_ROBOT_SPHERE: FN<SphereRobot>(__VA_ARGS__); break;    \
-        case PRRTC_ROBOT_PANDA: FN<PandaRobot>(__VA_ARGS__); break;      \
-        case PRRTC_ROBOT_FETCH: FN<FetchRobot>(__VA_ARGS__); break;      \
-        case PRRTC_ROBOT_BAXTER: FN<BaxterRobot>(__VA_ARGS__); break;    \
+// every robot of PRRTC_FOR_EACH_ROBOT (a variadic call does not pass
+// through that list's macro)
+#define PRRTC_DISPATCH(robot, FN, ...)                                    \
+    switch (robot) {                                                      \
+        case PRRTC_ROBOT_SPHERE: FN<SphereRobot>(__VA_ARGS__); break;     \
+        case PRRTC_ROBOT_PANDA: FN<PandaRobot>(__VA_ARGS__); break;       \
+        case PRRTC_ROBOT_FETCH: FN<FetchRobot>(__VA_ARGS__); break;       \
+        case PRRTC_ROBOT_BAXTER: FN<BaxterRobot>(__VA_ARGS__); break;     \
+        case PRRTC_ROBOT_UR5: FN<Ur5Robot>(__VA_ARGS__); break;           \
     }
 
 static bool all_finite(const float *p, size_t n) {

@@ -1,8 +1,11 @@
 """Robot model tables (joint tree, collision spheres, check structure).
 
-``<robot>.json`` is produced once by ``fit_model`` from the reference's
-generated FK code; see that module for the derivation.  Robot ids follow the
-reference's explicit instantiations (src/planning/pRRTC.cu:768-771).
+``<robot>.json`` for Panda, Fetch and Baxter is produced once by ``fit_model``
+from the reference's generated FK code; see that module for the derivation.
+Robot ids 0-3 follow the reference's explicit instantiations
+(src/planning/pRRTC.cu:768-771).  UR5 (id 4) is not in the reference: its
+table is compiled by ``urdf_compile`` from ``ur5.urdf`` and
+``ur5_spheres.json``.
 """
 from __future__ import annotations
 
@@ -14,8 +17,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-ROBOT_IDS = {"sphere": 0, "panda": 1, "fetch": 2, "baxter": 3}
-DOF = {"sphere": 3, "panda": 7, "fetch": 8, "baxter": 14}
+ROBOT_IDS = {"sphere": 0, "panda": 1, "fetch": 2, "baxter": 3, "ur5": 4}
+DOF = {"sphere": 3, "panda": 7, "fetch": 8, "baxter": 14, "ur5": 6}
 
 # The 3-DoF point robot: one sphere of radius 1 at q (Robots.hh:190-237,
 # utils.cuh:330-334); joint limits [-5, 5]^3.

@@ -53,7 +53,7 @@ def oracle_robots():
     from oracle import oracle
     from prrtc_amd.robots import load_model
     oracle.build()
-    return {r: oracle.OracleRobot(load_model(r)) for r in ("sphere",) + ROBOTS}
+    return {r: oracle.OracleRobot(load_model(r)) for r in ("sphere",) + ROBOTS + ("ur5",)}
 
 
 @pytest.fixture(scope="session")

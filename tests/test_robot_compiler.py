@@ -150,3 +150,85 @@ def test_rejects_outside_subset():
     with pytest.raises(ValueError, match="no limits"):
         U.compile_model('<robot name="x"><link name="a"/><link name="b"/><joint name="j" type="revolute">'
                         '<parent link="a"/><child link="b"/></joint></robot>', {"spheres": []})
+
+
+# ---------------------------------------------------------------- UR5
+# A fourth robot, not in the reference, compiled from a URDF of the UR5's
+# published kinematics and this project's sphere file.
+
+def ur5_inputs():
+    import json
+    with open(os.path.join(RDIR, "ur5.urdf")) as f:
+        urdf = f.read()
+    with open(os.path.join(RDIR, "ur5_spheres.json")) as f:
+        spec = json.load(f)
+    return urdf, spec
+
+
+def test_ur5_closed_form_tool_point():
+    """At the zero pose the tool frame sits at (0.81725, 0.19145, -0.005491)
+    (0.425 + 0.39225 out, 0.13585 - 0.1197 + 0.10915 + 0.0823 across,
+    0.089159 - 0.09465 up); the ee sphere is 0.04 m along the tool's x axis,
+    which points along +y there.  Turning the base joint by q turns it about z."""
+    urdf, spec = ur5_inputs()
+    m = U.compile_model(urdf, spec)
+    ee = next(i for i, s in enumerate(spec["spheres"]) if s["link"] == "ee_link")
+    tool = np.array([0.81725, 0.19145 + 0.04, 0.089159 - 0.09465])
+    np.testing.assert_allclose(U.model_fk(m, np.zeros((1, 6)))[0, ee], tool, atol=1e-9)
+    for q0 in (0.7, -2.1):
+        c, s = math.cos(q0), math.sin(q0)
+        want = np.array([c * tool[0] - s * tool[1], s * tool[0] + c * tool[1], tool[2]])
+        np.testing.assert_allclose(U.model_fk(m, np.array([[q0, 0, 0, 0, 0, 0]]))[0, ee], want, atol=1e-9)
+
+
+def test_ur5_compiled_chain_equals_urdf_composition():
+    urdf, spec = ur5_inputs()
+    m = U.compile_model(urdf, spec)
+    rng = np.random.default_rng(5)
+    lo, rg = np.array(m["lower"]), np.array(m["range"])
+    for q in lo + rg * rng.random((8, 6)):
+        poses = U.urdf_link_poses(urdf, q)
+        want = np.array([poses[s["link"]][:3, :3] @ s["xyz"] + poses[s["link"]][:3, 3] for s in spec["spheres"]])
+        np.testing.assert_allclose(U.model_fk(m, q)[0][:len(want)], want, atol=1e-12)
+
+
+def test_ur5_committed_table_and_header_are_the_compilers_output():
+    """robots/ur5.json is compile_model's output and csrc/robots/ur5.gen.h is
+    gen_headers' output for it (the device code of the fourth robot)."""
+    import json
+    from prrtc_amd.robots import gen_headers, load_model
+    urdf, spec = ur5_inputs()
+    m = U.compile_model(urdf, spec)
+    assert json.loads(json.dumps(m)) == load_model("ur5")
+    with open(os.path.join(ROOT, "prrtc_amd", "csrc", "robots", "ur5.gen.h")) as f:
+        assert f.read() == gen_headers.gen_robot("ur5")
+
+
+def test_ur5_oracle_verdicts_brute_force(oracle_robots):
+    """The oracle's table-driven checker on the compiled UR5 against a brute
+    force over every sphere of its env groups and self pairs (sphere
+    obstacles, float64 FK), away from contact."""
+    from oracle import oracle
+    from prrtc_amd import scenes as S
+    from prrtc_amd.robots import load_model
+    m = load_model("ur5")
+    R = oracle.OracleRobot(m)
+    rng = np.random.default_rng(11)
+    obst = np.c_[rng.uniform(-0.9, 0.9, (10, 2)), rng.uniform(0.0, 0.9, 10), rng.uniform(0.05, 0.15, 10)]
+    sc = S.Scene.build(spheres=[S.sphere(o[:3], o[3]) for o in obst])
+    lo, rg = np.array(m["lower"]), np.array(m["range"])
+    q = (lo + rg * rng.random((400, 6))).astype(np.float32)
+    free, margin = R.check(sc, q, margin=True)
+    c = U.model_fk(m, q.astype(np.float64))
+    rad = np.array([s["r"] for s in m["spheres"]])
+    env_sph = sorted({i for g in m["env_checks"] for i in (g["leaves"] or [g["gate"]])})
+    pairs = [g["gate"] if g["direct"] else None for g in m["self_checks"]]
+    assert all(p is not None for p in pairs)
+    d = np.linalg.norm(c[:, env_sph, None, :] - obst[None, None, :, :3], axis=-1) - rad[env_sph][None, :, None] - obst[None, None, :, 3]
+    hit_env = (d < 0).any(axis=(1, 2))
+    dp = np.stack([np.linalg.norm(c[:, a] - c[:, b], axis=-1) - rad[a] - rad[b] for a, b in pairs], axis=1)
+    hit_self = (dp < 0).any(axis=1)
+    brute_free = ~(hit_env | hit_self)
+    far = np.minimum(np.abs(d).min(axis=(1, 2)), np.abs(dp).min(axis=1)) > 1e-5
+    assert far.sum() > 300 and 0.1 < brute_free.mean() < 0.9, (far.sum(), brute_free.mean())
+    assert np.array_equal(free[far].astype(bool), brute_free[far])
