@@ -19,7 +19,8 @@ each step is one launch of the edge-validation kernel over all edges.
                  one persistent launch vs one launch per problem, and of the
                  same batch as graph-replayed round grids
     batched_per_robot  the same comparison for Panda and Baxter on the
-                 shelf problem sets of the mix (each class's settings)
+                 shelf problem sets of the mix (each class's settings), and
+                 for UR5, the robot built by the model compiler
     mix          128 Baxter + 256 Panda + 128 Fetch shelf problems (g 128,
                  dynamic domain, balance 1) in one call: one persistent grid
                  serves the three robots
@@ -85,7 +86,7 @@ def main():
                     help="max_iters of every problem of the batched vs plain comparisons (both forms)")
     ap.add_argument("--batch", type=int, default=256, help="Fetch problems of the batched vs plain comparison")
     ap.add_argument("--robot-batch", type=int, default=96,
-                    help="Panda and Baxter shelf problems of their batched vs plain comparisons")
+                    help="Panda and Baxter shelf problems, and UR5 problems, of their batched vs plain comparisons")
     ap.add_argument("--mix-scale", type=float, default=1.0, help="fraction of the 128/256/128 uneven mix")
     ap.add_argument("--serve-rate", type=float, default=500.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -446,14 +447,15 @@ def main():
         log(f"k_plan: {bstats['pose_checks']:.3e} poses, {out['k_plan_roofline']['achieved']:.2f} TFLOP/s "
             f"estimated ({out['k_plan_roofline']['frac'] * 100:.1f}% of packed peak)")
 
-        # ------------------------------ batched vs plain, Panda and Baxter
+        # ------------------------ batched vs plain, Panda, Baxter and UR5
         # The same comparison on the shelf problem sets of the uneven mix,
-        # one class at a time, with each class's settings and tuned nnc:
+        # and on UR5 problems (the robot the model compiler built), one class
+        # at a time, with each class's settings and tuned nnc:
         # on the problems a pilot batch connects (max_iters 20000 in both
         # forms), and on every problem under the round budget in both forms.
         from prrtc_amd.autotune import SETTINGS as CLASS_SETTINGS, problems as class_problems
         out["batched_per_robot"] = {}
-        for robot in ("panda", "baxter"):
+        for robot in ("panda", "baxter", "ur5"):
             rp = class_problems(robot, args.robot_batch)
             cbase = CLASS_SETTINGS[robot]
             nb_nnc = tuned.get("batch_nnc", {}).get(robot, 128)
@@ -476,7 +478,7 @@ def main():
                             "solved_plain": np_r, "plans_per_s_batched": nb_r / tb,
                             "plans_per_s_plain": np_r / tp, "speedup": (nb_r / tb) / max(1e-9, np_r / tp)}
                 log(f"{robot} {key}: batched {nb_r / tb:.1f} plans/s vs plain {np_r / tp:.1f} plans/s "
-                    f"({row[key]['speedup']:.2f}x) on {len(probs_)} shelf problems")
+                    f"({row[key]['speedup']:.2f}x) on {len(probs_)} problems")
                 if key == "connectable":
                     for p, r, pa in zip(probs_, rres, rpaths):
                         if r["solved"]:

@@ -4,7 +4,8 @@
 
 For every robot class it builds a seeded problem set of the benchmark's kind
 (Fetch: 8-64 mixed obstacles, g 64; Panda and Baxter: shelves with 4-128
-obstacles, g 128, dynamic domain), keeps the problems a pilot run connects,
+obstacles, g 128, dynamic domain; UR5: 12 mixed obstacles in reach, g 32,
+dynamic domain), keeps the problems a pilot run connects,
 and measures on the GPU:
 
 * batched form (one persistent launch): plans/s over num_new_configs x waves
@@ -50,6 +51,8 @@ SETTINGS = {
                   seed=1),
     "panda": dict(granularity=128, range=0.5, balance=1, tree_ratio=0.5, dynamic_domain=1, dd_radius=4.0,
                   dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, seed=1),
+    "ur5": dict(granularity=32, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=1, dd_radius=4.0,
+                dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, seed=1),
     "baxter": dict(granularity=128, range=0.5, balance=1, tree_ratio=0.5, dynamic_domain=1, dd_radius=4.0,
                    dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, seed=1),
 }
@@ -77,6 +80,8 @@ def problems(robot, n):
     from . import workloads
     if robot == "fetch":
         return workloads.c2_problems(n, seed=0)
+    if robot == "ur5":
+        return workloads.ur5_problems(n, seed=0)
     counts = {"panda": (0, n, 0), "baxter": (n, 0, 0)}[robot]
     return workloads.c3_problems(*counts, seed=0)
 

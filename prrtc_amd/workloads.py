@@ -5,6 +5,7 @@
     c1_edges     1M random edges, endpoints in the limits, length <= 0.5 rad
     c2_problems  256 Fetch problems, 8-64 mixed obstacles each          (batched)
     c3_problems  128 Baxter + 256 Panda + 128 Fetch, shelves + spheres   (uneven mix)
+    ur5_problems UR5 (the compiled fourth robot), 12 mixed obstacles in reach
     serving_request  one request of the Poisson serving stream
 
 Scenes come from prrtc_amd.scenes (SplitMix64-seeded).  Obstacles that
@@ -111,6 +112,34 @@ def c3_problems(n_baxter=128, n_panda=256, n_fetch=128, seed=0):
         for i in range(n):
             out.append(_problem(robot, lambda s: S.scene_shelf(s, 4, 128, avoid=avoid), seed + 1000 * len(out) + i))
     return out
+
+
+def ur5_scene(seed):
+    """6 spheres, 4 oriented cuboids and 2 capsules in the UR5's reach, none
+    within 5 cm of its base sphere."""
+    rng = np.random.default_rng(seed)
+    base = fixed_spheres("ur5")
+
+    def clear(c, r):
+        return bool(np.all(np.linalg.norm(base[:, :3] - c, axis=1) > base[:, 3] + r + 0.05))
+    sph, cub, cap = [], [], []
+    while len(sph) < 6:
+        c, r = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.0, 0.9)], rng.uniform(0.05, 0.12)
+        if clear(c, r):
+            sph.append(S.sphere(c, r))
+    while len(cub) < 4:
+        c, h = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.0, 0.8)], rng.uniform(0.03, 0.12, 3)
+        if clear(c, float(np.linalg.norm(h))):
+            cub.append(S.cuboid(c, rng.uniform(-np.pi, np.pi, 3), h))
+    while len(cap) < 2:
+        c = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.1, 0.8)]
+        if clear(c, 0.2):
+            cap.append(S.cylinder_center(c, rng.uniform(-np.pi, np.pi, 3), 0.05, 0.3))
+    return S.Scene.build(spheres=sph, capsules=cap, cuboids=cub)
+
+
+def ur5_problems(n, seed=0):
+    return [_problem("ur5", ur5_scene, seed + 1000 * i) for i in range(n)]
 
 
 def serving_request(i, seed=0):

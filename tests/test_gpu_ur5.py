@@ -13,31 +13,6 @@ from test_gpu_planner import _check_path, assert_same_result
 pytestmark = pytest.mark.gpu
 
 
-def ur5_scene(seed):
-    """Spheres, oriented cuboids and capsules in the UR5's reach, none
-    touching its base."""
-    from prrtc_amd import scenes as S
-    from prrtc_amd.robots import fixed_spheres
-    rng = np.random.default_rng(seed)
-    base = fixed_spheres("ur5")
-    def clear(c, r):
-        return np.all(np.linalg.norm(base[:, :3] - c, axis=1) > base[:, 3] + r + 0.05)
-    sph, cub, cap = [], [], []
-    while len(sph) < 6:
-        c, r = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.0, 0.9)], rng.uniform(0.05, 0.12)
-        if clear(c, r):
-            sph.append(S.sphere(c, r))
-    while len(cub) < 4:
-        c, h = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.0, 0.8)], rng.uniform(0.03, 0.12, 3)
-        if clear(c, float(np.linalg.norm(h))):
-            cub.append(S.cuboid(c, rng.uniform(-np.pi, np.pi, 3), h))
-    while len(cap) < 2:
-        c = np.r_[rng.uniform(-0.8, 0.8, 2), rng.uniform(0.1, 0.8)]
-        if clear(c, 0.2):
-            cap.append(S.cylinder_center(c, rng.uniform(-np.pi, np.pi, 3), 0.05, 0.3))
-    return S.Scene.build(spheres=sph, capsules=cap, cuboids=cub)
-
-
 def test_ur5_fk(gpu):
     from prrtc_amd.robots import load_model
     from prrtc_amd.robots.urdf_compile import model_fk
@@ -53,7 +28,8 @@ def test_ur5_fk(gpu):
 def test_ur5_pose_and_edge_verdicts(seed, gpu, oracle_robots):
     from prrtc_amd.robots import limits
     R = oracle_robots["ur5"]
-    sc = ur5_scene(seed)
+    from prrtc_amd import workloads
+    sc = workloads.ur5_scene(seed)
     lo, span = limits("ur5")
     rng = np.random.default_rng(seed + 10)
     q = (lo + span * rng.random((4096, 6), dtype=np.float32)).astype(np.float32)
@@ -76,7 +52,7 @@ def test_ur5_pose_and_edge_verdicts(seed, gpu, oracle_robots):
 
 def _ur5_problems(n):
     from prrtc_amd import workloads
-    return [workloads._problem("ur5", ur5_scene, 100 + i) for i in range(n)]
+    return workloads.ur5_problems(n, seed=100)
 
 
 def test_ur5_plans_alone_batched_and_mixed(gpu, oracle_robots):
