@@ -175,7 +175,9 @@ typedef struct prrtc_batch_stats {
 /* settings[i] applies to problem i (settings may also point to one struct
  * when n_settings == 1).  paths: one block of path_cap * 14 floats per
  * problem; block i holds results[i].path_nodes rows of that problem's dof
- * floats, packed, goal -> start. */
+ * floats, packed, goal -> start.  The problems may have different robots:
+ * one persistent grid serves them all, and each result equals that
+ * problem's prrtc_solve. */
 int prrtc_solve_batch(int n_problems, const prrtc_problem *problems, const prrtc_settings *settings,
                       int n_settings, prrtc_result *results, float *paths, int path_cap,
                       prrtc_batch_stats *stats);

@@ -1193,7 +1193,8 @@ struct GroupRun {
 };
 
 // Upload a group, launch its persistent grid and queue the state readback
-// on the group's own stream (groups of different robots overlap).
+// on the group's own stream (a mixed batch is one group; per-robot groups,
+// where used, overlap on the device).
 static void start_group(GroupRun &gr, int path_cap) {
     const std::vector<HostProblem> &probs = gr.probs;
     const int P = (int)probs.size();
