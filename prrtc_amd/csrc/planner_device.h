@@ -335,9 +335,11 @@ __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const 
     const int lane = lane_id();
     const int W = tm.waves, wv = tm.wave;
     // kScanChunk-node chunks, chunk k to wave k % W: lane l reads nodes
-    // c + 256u + 4l .. +3 (u = 0..3) of every column as float4, so 4*D 16-byte
-    // loads are in flight per lane.  The last chunk is read whole too (the
-    // tree's capacity is a multiple of kScanChunk) and nodes >= n are masked.
+    // c + 256u + 4l .. +3 (u = 0..3) of every column as float4, four 16-byte
+    // loads per coordinate (the compiled loop waits for each coordinate's
+    // loads before the next coordinate's).  The last chunk is read whole too
+    // (the tree's capacity is a multiple of kScanChunk) and nodes >= n are
+    // masked.
     // Each lane visits its nodes in increasing index order.
     static_assert(kScanChunk == 16 * kWave, "scan chunk = 16 nodes per lane");
     for (int c = wv * kScanChunk; c < n; c += W * kScanChunk) {
