@@ -396,5 +396,15 @@ PRRTC_GEN_ROBOT(Ur5, ur5)
     X(PRRTC_ROBOT_BAXTER, BaxterRobot) \
     X(PRRTC_ROBOT_UR5, Ur5Robot)
 constexpr int kNumRobots = 5;   // ids 0 .. kNumRobots - 1
+constexpr bool robot_ids_dense() {
+    int n = 0, top = -1;
+#define PRRTC_COUNT_ROBOT(ID, R) n++; top = ID > top ? ID : top;
+    PRRTC_FOR_EACH_ROBOT(PRRTC_COUNT_ROBOT)
+#undef PRRTC_COUNT_ROBOT
+    return n == kNumRobots && top == kNumRobots - 1;
+}
+static_assert(robot_ids_dense(), "PRRTC_FOR_EACH_ROBOT lists ids 0 .. kNumRobots - 1");
+// The type of a robot in a call expanded from PRRTC_FOR_EACH_ROBOT.
+template <class R> struct RobotTag { using type = R; };
 
 }  // namespace prrtc

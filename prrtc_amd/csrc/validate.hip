@@ -239,16 +239,15 @@ static void launch_fk(const float *d_poses, int n, float *d_out, hipStream_t st)
     PRRTC_HIP(hipGetLastError());
 }
 
-// every robot of PRRTC_FOR_EACH_ROBOT (a variadic call does not pass
-// through that list's macro)
-#define PRRTC_DISPATCH(robot, FN, ...)                                    \
-    switch (robot) {                                                      \
-        case PRRTC_ROBOT_SPHERE: FN<SphereRobot>(__VA_ARGS__); break;     \
-        case PRRTC_ROBOT_PANDA: FN<PandaRobot>(__VA_ARGS__); break;       \
-        case PRRTC_ROBOT_FETCH: FN<FetchRobot>(__VA_ARGS__); break;       \
-        case PRRTC_ROBOT_BAXTER: FN<BaxterRobot>(__VA_ARGS__); break;     \
-        case PRRTC_ROBOT_UR5: FN<Ur5Robot>(__VA_ARGS__); break;           \
-    }
+// FN<Robot>(args...) for the robot id, over every robot of
+// PRRTC_FOR_EACH_ROBOT (the arguments reach each case through a lambda)
+#define PRRTC_DISPATCH_CASE(ID, R) \
+    case ID: call_(RobotTag<R>{}); break;
+#define PRRTC_DISPATCH(robot, FN, ...)                                              \
+    do {                                                                            \
+        auto call_ = [&](auto tag) { FN<typename decltype(tag)::type>(__VA_ARGS__); }; \
+        switch (robot) { PRRTC_FOR_EACH_ROBOT(PRRTC_DISPATCH_CASE) }                 \
+    } while (0)
 
 static bool all_finite(const float *p, size_t n) {
     for (size_t i = 0; i < n; i++)
