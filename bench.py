@@ -12,9 +12,11 @@ each step is one launch of the edge-validation kernel over all edges.
                  same edges (every pose fully evaluated): reference flops /
                  kernel time against MI355X's FP32 vector peak
     plan         single-problem plans of the same scene (g=128, dynamic
-                 domain): p50/p99 solve ms
+                 domain): p50/p99 solve ms (the library's clock, API call ->
+                 result; the Python call around it beside it)
     c0           plain Panda problem (12 spheres, nnc 128, g 32) over seeds:
-                 p50/p99 solve ms, success, cost; CPU RRT-Connect beside it
+                 p50/p99 solve ms (same clock), success, cost; CPU
+                 RRT-Connect beside it
     batched      256 Fetch problems (8-64 mixed obstacles, g 64): plans/s of
                  one persistent launch vs one launch per problem, and of the
                  same batch as graph-replayed round grids
@@ -312,15 +314,20 @@ def main():
         st = dict(num_new_configs=128, granularity=128, range=0.5, balance=2, tree_ratio=1.0, dynamic_domain=1,
                   dd_radius=4.0, dd_alpha=1e-4, dd_min_radius=1.0, max_samples=100000, max_iters=20000)
         P.solve("panda", probs[0]["start"], probs[0]["goals"], scene, dict(st, seed=0))   # warm
-        times, solved = [], 0
+        # solve time = the library's own clock, API call -> result
+        # (prrtc_result.wall_ns, the reference's measure, pRRTC.cu:615, 763);
+        # the Python call around it (argument packing, ctypes) beside it
+        times, py_times, solved = [], [], 0
         for i, p in enumerate(probs):
             t0 = time.perf_counter()
             res, path = P.solve("panda", p["start"], p["goals"], p["scene"], dict(st, seed=i))
-            times.append((time.perf_counter() - t0) * 1e3)
+            py_times.append((time.perf_counter() - t0) * 1e3)
+            times.append(res["wall_ns"] / 1e6)
             solved += res["solved"]
             if res["solved"]:
                 paths_to_check.append(("panda", p["scene"], path, 128))
         out["plan"] = {"problems": len(probs), "solved": solved, "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
+                       "python_call_p50_ms": pct(py_times, 50),
                        "settings": "panda, nnc 128, g 128, dd on (r 4, alpha 1e-4, min 1), range 0.5, balance 2"}
         log(f"plans: {solved}/{len(probs)} solved, p50 {out['plan']['p50_ms']:.1f} ms, "
             f"p99 {out['plan']['p99_ms']:.1f} ms")
@@ -333,11 +340,12 @@ def main():
                      max_samples=100000, max_iters=args.c0_rounds)
         c0_probs = [workloads.c0_problem(s) for s in range(args.c0_seeds)]
         P.solve("panda", c0_probs[0]["start"], c0_probs[0]["goals"], c0_probs[0]["scene"], c0_st)   # warm
-        times, costs, rounds, c0_res = [], [], [], []
+        times, py_times, costs, rounds, c0_res = [], [], [], [], []
         for seed, p in enumerate(c0_probs):
             t0 = time.perf_counter()
             res, path = P.solve("panda", p["start"], p["goals"], p["scene"], dict(c0_st, seed=seed))
-            times.append((time.perf_counter() - t0) * 1e3)
+            py_times.append((time.perf_counter() - t0) * 1e3)
+            times.append(res["wall_ns"] / 1e6)   # the library's clock, as for the plans above
             c0_res.append(res)
             if res["solved"]:
                 costs.append(res["cost"])
@@ -345,7 +353,7 @@ def main():
                 paths_to_check.append(("panda", p["scene"], path, 32))
         solved = len(costs)
         out["c0"] = {"problems": len(c0_probs), "solved": solved, "success_rate": solved / len(c0_probs),
-                     "p50_ms": pct(times, 50), "p99_ms": pct(times, 99),
+                     "p50_ms": pct(times, 50), "p99_ms": pct(times, 99), "python_call_p50_ms": pct(py_times, 50),
                      "cost_p10": pct(costs, 10), "cost_p50": pct(costs, 50), "cost_p90": pct(costs, 90),
                      "rounds_p50": pct(rounds, 50), "rounds_p90": pct(rounds, 90),
                      "settings": f"panda, 12 spheres, problem and planner seed i for i < {len(c0_probs)}, nnc 128, "

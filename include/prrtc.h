@@ -85,7 +85,8 @@ typedef struct prrtc_result {
     int64_t kernel_ns;        /* device time of the solve (pRRTC.cu:696-706) */
     int64_t copy_ns;          /* host<->device copies (pRRTC.cu:685-694, 709-713) */
     int64_t edge_checks;      /* edges whose verdict was decided */
-    int64_t pose_checks;      /* poses put on lanes for those edges (one pose per lane and pass) */
+    int64_t pose_checks;      /* poses put on lanes for those edges (one pose per lane and pass;
+                                 like the times, it depends on the launch shape) */
 } prrtc_result;
 
 /* Fill `s` with the reference's defaults (pRRTC_settings.hh:4-16), seed 1. */

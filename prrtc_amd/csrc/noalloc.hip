@@ -107,7 +107,7 @@ int solve_noalloc(const HostProblem &hp, prrtc_result *out, float *path, int pat
     PRRTC_HIP(hipMemcpyAsync(blk + L.init + sizeof(float) * D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D,
                              hipMemcpyHostToDevice, s));
     PlanDesc desc;
-    fill_desc(desc, hp, L, blk);
+    fill_desc(desc, hp, L, blk, true);
     PlanControl ctrl{};
     ctrl.n_problems = 1;
     ctrl.mask_words = 1;

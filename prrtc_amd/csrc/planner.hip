@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <climits>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -68,7 +69,8 @@ __device__ __forceinline__ int *task_stop_words() {
 struct TaskCtx : CheckCtx {
     uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
     int pass;         // index of the current pass within the task
-    int waves;        // waves of the task: the whole workgroup, or 1
+    int waves;        // waves sharing this pass's poses (1: the wave has poses of its own)
+    bool shared_stop; // the task runs on several waves: stop keys go through LDS
     unsigned mine12;  // bit i: self groups j with j mod 12 == i are this wave's (12 = lcm of 1..4)
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
         const int W = waves;
@@ -109,7 +111,7 @@ struct TaskCtx : CheckCtx {
     __device__ __forceinline__ bool stop() {
         const uint64_t cm = ballot(coll);
         int sk = cm ? __builtin_amdgcn_readlane(key, __builtin_ctzll(cm)) : 0x7fffffff;
-        if (waves > 1) {
+        if (shared_stop) {
             int *w = task_stop_words() + (pass & 1);
             if (cm && lane_id() == 0) __hip_atomic_fetch_min(w, sk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             sk = min(sk, __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
@@ -142,12 +144,27 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     // (through the caller's reference they would be reloaded after every
     // store to the pass flags)
     TaskCtx ctx = ctx_in;
+    const Obstacles full = ctx_in.obs;   // the whole staged scene
+    unsigned mine_split;                  // self groups of this wave when the waves share poses
     {
         unsigned m = 0;
         for (int i = 0; i < 12; i++) m |= (unsigned)(i % tm.waves == tm.wave) << i;
         // uniform over the wave: the group branches stay scalar branches
-        ctx.mine12 = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
+        mine_split = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
     }
+    ctx.shared_stop = tm.waves > 1;
+    // A pass either puts the same 64 poses on every wave, which split the
+    // obstacles and self groups (short work: one edge, a chain's first
+    // sub-edges), or, with `wide`, 64 poses of its own on each wave with the
+    // whole scene (long chains: no check is computed twice, no gate
+    // exchange).  Either way every pose gets the full check and a collision
+    // on any wave retires the lanes at or above its key on all of them.
+    auto shape = [&](bool wide) {
+        ctx.obs = wide || tm.waves == 1 ? full : full.slice(tm.wave, tm.waves);
+        ctx.waves = wide ? 1 : tm.waves;
+        ctx.mine12 = wide ? 0xfffu : mine_split;
+    };
+    shape(false);
     constexpr int D = Robot::DOF;
     const int tid = tm.tid, NT = tm.nt;
     const int lane = lane_id();
@@ -209,7 +226,8 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
         const int passes = (g + kWave - 1) / kWave;
-        int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0;
+        int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0, cpass = 0;
+        bool wide = false;
         unsigned n_ext = 0;
         edges = 1;
         while (true) {
@@ -223,7 +241,12 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
                 for (int c = 0; c < D; c++) q[c] = fmaf(kk, delta[c], nnv[c]);
                 dead = k >= g;
             } else {
-                const int j = base + lane;
+                // from chain pass wide_from on, while more than one wave's
+                // worth of poses remains, wave w takes poses base + 64 w + lane
+                const bool w_next = tm.waves > 1 && d.wide_from >= 0 && cpass >= d.wide_from && total - base > kWave;
+                if (w_next != wide) shape(w_next);
+                wide = w_next;
+                const int j = base + (wide ? tm.wave * kWave : 0) + lane;
                 const int jj = j < total ? j : total - 1;
                 const int i = jj / g, k = jj % g;
 #pragma unroll
@@ -235,7 +258,8 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
             ctx.dead = dead;
             ctx.key = key;
             ctx.pass = npass;
-            poses += __builtin_popcountll(ballot(!dead));
+            // poses of the team's pass (the same count on every wave)
+            poses += phase == 0 ? __builtin_popcountll(ballot(!dead)) : min(total - base, wide ? kWave * tm.waves : kWave);
             // the other parity's word is next read in the following pass,
             // after this pass's closing barrier
             if (tid == 0 && tm.waves > 1) stopw[(npass + 1) & 1] = 0x7fffffff;
@@ -249,7 +273,8 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
                     n_ok = first;
                     break;
                 }
-                base += kWave;
+                base += wide ? kWave * tm.waves : kWave;
+                cpass++;
                 if (base >= total) { n_ok = n_chain; break; }
                 continue;
             }
@@ -756,8 +781,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             }
             __threadfence();
         } else if (runs) {
-            ctx.obs = crowd ? scene_obs : scene_obs.slice(w, W);
-            ctx.waves = crowd ? 1 : W;
+            ctx.obs = scene_obs;
             float *a = areas + (crowd ? (size_t)w * area : 0);
             run_task<Robot>(d, st, crowd ? s + w : s, ctx, a, lds_ext, crowd ? team_wave() : team_workgroup());
             __threadfence();
@@ -838,9 +862,7 @@ void k_round_tasks(const PlanDesc *__restrict__ descs, PlanState *states, int ma
     TaskCtx ctx;
     ctx.independent = false;
     ctx.gbuf = gbuf;
-    ctx.waves = blockDim.x / kWave;
     bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
-    ctx.obs = ctx.obs.slice(wave_id(), ctx.waves);
     plan_task<Robot>(d, st, s, ctx, a, a + (lds_ext + 1) * D, a + (lds_ext + 5) * D,
                      reinterpret_cast<int *>(a + (lds_ext + 5) * D + 2 * kMaxTaskWaves), team_workgroup());
 }
@@ -987,8 +1009,24 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
     std::memcpy(ini + D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D);
 }
 
-void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev) {
+// Chain pass from which a task's waves take distinct poses (negative:
+// never).  A problem alone whose round fits one task per CU waits for its
+// slowest task, so every chain pass goes wide; otherwise the first chain pass
+// shares its 64 poses (many chains collide in their first sub-edges, and a
+// shared pass ends soonest there; with more tasks than CUs the extra poses
+// of a wide first pass cost throughput).  PRRTC_WIDE_FROM overrides.
+static int wide_from(int nnc, bool alone) {
+    static const int env = [] {
+        const char *e = std::getenv("PRRTC_WIDE_FROM");
+        return e ? std::atoi(e) : INT_MIN;
+    }();
+    if (env != INT_MIN) return env;
+    return alone && nnc <= num_cus() ? 0 : 1;
+}
+
+void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev, bool alone) {
     d = PlanDesc{};
+    d.wide_from = wide_from(hp.st->num_new_configs, alone);
     d.nnc = hp.st->num_new_configs;
     d.g = hp.st->granularity;
     d.max_iters = hp.st->max_iters;
@@ -1244,7 +1282,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     char *host = ws.need_host(hl.off);
     PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + h_hdr + o_desc);
     for (int i = 0; i < P; i++) {
-        fill_desc(hd[i], probs[i], lay[i], dev + o_block[i]);
+        fill_desc(hd[i], probs[i], lay[i], dev + o_block[i], P == 1);
         block_upload_image(probs[i], lay[i], host + h_block[i]);
     }
     PlanControl *hc = reinterpret_cast<PlanControl *>(host + h_hdr + o_ctrl);

@@ -37,7 +37,7 @@ struct PlanDesc {
     int balance, dd, n_goals, cap;
     int max_ext, scene_f4;
     int robot;            // PRRTC_ROBOT_*: the planner code a mixed launch dispatches to
-    int pad1;
+    int wide_from;        // chain pass from which a task's waves take distinct poses (< 0: never)
     float range, tree_ratio, dd_alpha, dd_radius;
     float dd_min_radius, pad2, pad3, pad4;
     const float *scene;   // packed scene
@@ -330,8 +330,11 @@ template <int D>
 __device__ __forceinline__ int nearest(const float *tree, int cap, int n, const float (&q)[D], float &best_sq,
                                        float *red, const Team &tm) {
 #pragma clang fp contract(off)
+    // (best, idx) starts at (+inf, 0): a finite distance always wins, and a
+    // query whose distances all overflow to +inf (or are NaN) still gets a
+    // node of the tree, never an index past it
     float best = __builtin_huge_valf();
-    int idx = 0x7fffffff;
+    int idx = 0;
     const int lane = lane_id();
     const int W = tm.waves, wv = tm.wave;
     // kScanChunk-node chunks, chunk k to wave k % W: lane l reads nodes

@@ -323,7 +323,7 @@ void ServerImpl::launch(RobotClass &c) {
             block_upload_image(hp, rq->lay, stg + so);
             PRRTC_HIP(hipMemcpyAsync(arena_dev + rq->off, stg + so, rq->lay.upload_bytes, hipMemcpyHostToDevice, s));
             so += (rq->lay.upload_bytes + 255) & ~size_t(255);
-            fill_desc(c.h_desc[rq->entry], hp, rq->lay, arena_dev + rq->off);
+            fill_desc(c.h_desc[rq->entry], hp, rq->lay, arena_dev + rq->off, false);
             c.h_list[i] = rq->entry;
             lo = std::min(lo, rq->entry);
             hi = std::max(hi, rq->entry);

@@ -49,3 +49,18 @@ def test_nearest_matches_brute_force(dof, n, gpu):
             assert dist[i] <= dist.min() * (1 + 1e-5) + 1e-6
             assert abs(d2[t, j] - dist[i]) <= 1e-5 * max(1.0, dist[i])
 
+
+
+def test_nearest_overflowing_distances(gpu):
+    """Distances that overflow float32 (nodes at 3e19, squares past FLT_MAX)
+    or are NaN (a NaN query) compare below no bound: the answer is node 0
+    with +inf, an index inside the tree, never one past it."""
+    dof, n, cap = 7, 1500, 2048
+    trees = np.full((2, dof, cap), np.nan, np.float32)
+    trees[:, :, :n] = 3e19
+    queries = np.zeros((2, 3, dof), np.float32)
+    queries[1, 1, 2] = np.nan
+    for w in (1, 2, 4):
+        idx, d2 = run(gpu, trees, n, queries, w)
+        assert np.array_equal(idx, np.zeros_like(idx)), (w, idx)
+        assert np.all(np.isinf(d2)), (w, d2)
