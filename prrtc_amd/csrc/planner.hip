@@ -70,6 +70,7 @@ struct TaskCtx : CheckCtx {
     uint64_t *gbuf;   // LDS, kMaxTaskWaves * kWave words
     int pass;         // index of the current pass within the task
     int waves;        // waves sharing this pass's poses (1: the wave has poses of its own)
+    int gbase;        // first wave of those
     bool shared_stop; // the task runs on several waves: stop keys go through LDS
     unsigned mine12;  // bit i: self groups j with j mod 12 == i are this wave's (12 = lcm of 1..4)
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
@@ -79,7 +80,7 @@ struct TaskCtx : CheckCtx {
         __syncthreads();
         uint64_t g = 0;
         const int lane = lane_id();
-        for (int w = 0; w < W; w++) g |= gbuf[w * kWave + lane];
+        for (int w = 0; w < W; w++) g |= gbuf[(gbase + w) * kWave + lane];
         __syncthreads();   // gbuf is written again by the next pass
         gate = g;
     }
@@ -145,26 +146,38 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     // store to the pass flags)
     TaskCtx ctx = ctx_in;
     const Obstacles full = ctx_in.obs;   // the whole staged scene
-    unsigned mine_split;                  // self groups of this wave when the waves share poses
-    {
-        unsigned m = 0;
-        for (int i = 0; i < 12; i++) m |= (unsigned)(i % tm.waves == tm.wave) << i;
-        // uniform over the wave: the group branches stay scalar branches
-        mine_split = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
-    }
     ctx.shared_stop = tm.waves > 1;
-    // A pass either puts the same 64 poses on every wave, which split the
-    // obstacles and self groups (short work: one edge, a chain's first
-    // sub-edges), or, with `wide`, 64 poses of its own on each wave with the
-    // whole scene (long chains: no check is computed twice, no gate
-    // exchange).  Either way every pose gets the full check and a collision
-    // on any wave retires the lanes at or above its key on all of them.
-    auto shape = [&](bool wide) {
-        ctx.obs = wide || tm.waves == 1 ? full : full.slice(tm.wave, tm.waves);
-        ctx.waves = wide ? 1 : tm.waves;
-        ctx.mine12 = wide ? 0xfffu : mine_split;
+    // A pass puts 64 poses on each of G pose groups of the task's waves (G =
+    // 1, 2 or all of them).  A group's waves test the same poses and split
+    // the obstacles and self groups between them, which ends short work
+    // soonest (one edge, a chain's first sub-edges); a group of one wave
+    // tests its poses against the whole scene, which computes no check twice
+    // and needs no gate exchange (long chains).  Every pose gets the full
+    // check whatever the shape, and a collision on any wave retires the
+    // lanes at or above its key on all of them.
+    int groups = 0;   // G of the current shape
+    int gw = 0;       // this wave's pose group
+    auto shape = [&](int G) {
+        if (G == groups) return;
+        groups = G;
+        const int gs = tm.waves / G;          // waves per group
+        gw = tm.wave / gs;
+        const int wi = tm.wave - gw * gs;     // this wave's index in its group
+        ctx.obs = gs == 1 ? full : full.slice(wi, gs);
+        ctx.waves = gs;
+        ctx.gbase = gw * gs;
+        unsigned m = 0;
+        for (int i = 0; i < 12; i++) m |= (unsigned)(i % gs == wi) << i;
+        // uniform over the wave: the group branches stay scalar branches
+        ctx.mine12 = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
     };
-    shape(false);
+    // pose groups for n poses: one pass where the waves allow it
+    auto groups_for = [&](int n) {
+        const int want = (n + kWave - 1) / kWave;
+        if (want <= 1 || tm.waves == 1) return 1;
+        if (want == 2 && (tm.waves & 1) == 0) return 2;
+        return tm.waves;
+    };
     constexpr int D = Robot::DOF;
     const int tid = tm.tid, NT = tm.nt;
     const int lane = lane_id();
@@ -225,9 +238,10 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
         tm.sync();
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
-        const int passes = (g + kWave - 1) / kWave;
+        const int G0 = d.wide_edge ? groups_for(g) : 1;
+        const int passes = (g + kWave * G0 - 1) / (kWave * G0);
+        shape(G0);
         int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0, cpass = 0;
-        bool wide = false;
         unsigned n_ext = 0;
         edges = 1;
         while (true) {
@@ -235,18 +249,18 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
             int key = 0;
             bool dead;
             if (phase == 0) {
-                const int k = lane * passes + p;
+                // virtual lane (lane, group) interleaved, so every pass of
+                // every group spans the whole edge
+                const int k = (lane * G0 + gw) * passes + p;
                 const float kk = (float)((k < g ? k : g - 1) + 1);
 #pragma unroll
                 for (int c = 0; c < D; c++) q[c] = fmaf(kk, delta[c], nnv[c]);
                 dead = k >= g;
             } else {
-                // from chain pass wide_from on, while more than one wave's
-                // worth of poses remains, wave w takes poses base + 64 w + lane
-                const bool w_next = tm.waves > 1 && d.wide_from >= 0 && cpass >= d.wide_from && total - base > kWave;
-                if (w_next != wide) shape(w_next);
-                wide = w_next;
-                const int j = base + (wide ? tm.wave * kWave : 0) + lane;
+                // from chain pass wide_from on, the remaining poses spread
+                // over pose groups: group gw takes poses base + 64 gw + lane
+                shape(d.wide_from >= 0 && cpass >= d.wide_from ? groups_for(total - base) : 1);
+                const int j = base + gw * kWave + lane;
                 const int jj = j < total ? j : total - 1;
                 const int i = jj / g, k = jj % g;
 #pragma unroll
@@ -259,7 +273,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
             ctx.key = key;
             ctx.pass = npass;
             // poses of the team's pass (the same count on every wave)
-            poses += phase == 0 ? __builtin_popcountll(ballot(!dead)) : min(total - base, wide ? kWave * tm.waves : kWave);
+            poses += phase == 0 ? min(kWave * G0, (g - 1 - p) / passes + 1) : min(total - base, kWave * groups);
             // the other parity's word is next read in the following pass,
             // after this pass's closing barrier
             if (tid == 0 && tm.waves > 1) stopw[(npass + 1) & 1] = 0x7fffffff;
@@ -273,7 +287,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
                     n_ok = first;
                     break;
                 }
-                base += wide ? kWave * tm.waves : kWave;
+                base += kWave * groups;
                 cpass++;
                 if (base >= total) { n_ok = n_chain; break; }
                 continue;
@@ -1024,9 +1038,20 @@ static int wide_from(int nnc, bool alone) {
     return alone && nnc <= num_cus() ? 0 : 1;
 }
 
+// Whether a steered edge of more than 64 poses spreads them over pose groups
+// (PRRTC_WIDE_EDGE overrides).
+static int wide_edge() {
+    static const int v = [] {
+        const char *e = std::getenv("PRRTC_WIDE_EDGE");
+        return e ? std::atoi(e) : 1;
+    }();
+    return v;
+}
+
 void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev, bool alone) {
     d = PlanDesc{};
     d.wide_from = wide_from(hp.st->num_new_configs, alone);
+    d.wide_edge = wide_edge();
     d.nnc = hp.st->num_new_configs;
     d.g = hp.st->granularity;
     d.max_iters = hp.st->max_iters;

@@ -39,7 +39,9 @@ struct PlanDesc {
     int robot;            // PRRTC_ROBOT_*: the planner code a mixed launch dispatches to
     int wide_from;        // chain pass from which a task's waves take distinct poses (< 0: never)
     float range, tree_ratio, dd_alpha, dd_radius;
-    float dd_min_radius, pad2, pad3, pad4;
+    float dd_min_radius;
+    int wide_edge;        // a steered edge of more than 64 poses spreads them over pose groups (0: never)
+    float pad3, pad4;
     const float *scene;   // packed scene
     float *nodes;         // [2][D][cap]   (SoA per tree)
     int *parent;          // [2][cap]
