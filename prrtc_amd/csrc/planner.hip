@@ -383,6 +383,101 @@ __device__ int trace(const PlanDesc &d, int t, int cur, float *out, int rs, floa
     return n;
 }
 
+// The rows and segment lengths of the first (up to 64) rows of one walk,
+// held one node per lane in `hop` (row j on lane j); `next` is the node after
+// row 63 (a longer walk) and `root` the walk's root.  Rows go to `out`, and
+// lane j's segment length |node_j - node_{j+1}| is returned.
+template <int D>
+__device__ float trace_rows(const PlanDesc &d, int t, int hop, int len, int next, int root, float *out, int rs) {
+#pragma clang fp contract(off)
+    const int lane = lane_id();
+    const float *tn = d.nodes + (size_t)t * D * d.cap;
+    const int up = __shfl_down(hop, 1);
+    const int p = lane + 1 < len ? (lane + 1 < kWave ? up : next) : root;
+    float acc = 0.0f;
+    if (lane < len) {
+        for (int k = 0; k < D; k++) {
+            const float a = tn[(size_t)k * d.cap + hop];
+            const float diff = a - tn[(size_t)k * d.cap + p];
+            acc = fmaf(diff, diff, acc);
+            out[(size_t)lane * rs + k] = a;
+        }
+    }
+    return sqrtf(acc);
+}
+
+// Trace the connection of a round (pRRTC.cu:552-582) with the whole wave:
+// lane 0 walks tree t from its new node and lane 1 the other tree from its
+// nearest node, so every hop is one dependent load for both walks, and the
+// first 64 nodes of each walk stay in registers (row j on lane j).  After the
+// rows are reserved, those rows are written and their segments measured in
+// parallel; the cost is summed in walk order (tree t's segments, then the
+// other tree's), the order of a sequential trace, and a walk longer than 64
+// rows continues on lane 0 from its 65th node.  Lane 0 holds the cost and
+// writes the state.
+template <int D>
+__device__ void trace_connection(const PlanDesc &d, PlanState *st, PlanControl *ctrl, float *paths, int t,
+                                 int cur_t, int cur_o) {
+#pragma clang fp contract(off)
+    const int lane = lane_id();
+    const int *par = d.parent + (size_t)(lane == 0 ? t : 1 - t) * d.cap;
+    int cur = lane == 0 ? cur_t : cur_o;
+    bool walking = lane < 2;
+    int len = 0;
+    int hop_t = 0, hop_o = 0, next_t = -1, next_o = -1;
+    while (true) {
+        const int nxt = walking ? par[cur] : cur;
+        const bool row = walking && nxt != cur;   // cur is not the root: a row of its walk
+        const uint64_t m = ballot(row);
+        if (m == 0) break;
+        const int c0 = __builtin_amdgcn_readlane(cur, 0), c1 = __builtin_amdgcn_readlane(cur, 1);
+        const int l0 = __builtin_amdgcn_readlane(len, 0), l1 = __builtin_amdgcn_readlane(len, 1);
+        if (m & 1) {
+            if (lane == l0) hop_t = c0;
+            if (l0 == kWave) next_t = c0;
+        }
+        if (m & 2) {
+            if (lane == l1) hop_o = c1;
+            if (l1 == kWave) next_o = c1;
+        }
+        if (row) {
+            len++;
+            cur = nxt;
+        } else {
+            walking = false;
+        }
+    }
+    const int tl = __builtin_amdgcn_readlane(len, 0), ol = __builtin_amdgcn_readlane(len, 1);
+    const int root_t = __builtin_amdgcn_readlane(cur, 0), root_o = __builtin_amdgcn_readlane(cur, 1);
+    int off = 0;
+    if (lane == 0) off = atomic_add_agent(&ctrl->path_rows, tl + ol);
+    off = __builtin_amdgcn_readfirstlane(off);
+    if (off + tl + ol > ctrl->path_cap) {
+        if (lane == 0) st->status = kStatusPathLimit;
+        return;
+    }
+    const int rs = ctrl->path_stride > 0 ? ctrl->path_stride : D;
+    float *out_t = paths + (size_t)off * rs, *out_o = paths + (size_t)(off + tl) * rs;
+    const float seg_t = trace_rows<D>(d, t, hop_t, tl, next_t, root_t, out_t, rs);
+    const float seg_o = trace_rows<D>(d, 1 - t, hop_o, ol, next_o, root_o, out_o, rs);
+    float cost = 0.0f;
+    int root;
+    for (int j = 0; j < min(tl, kWave); j++) cost += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(seg_t), j));
+    if (tl > kWave && lane == 0) trace<D>(d, t, next_t, out_t + (size_t)kWave * rs, rs, &cost, &root);
+    for (int j = 0; j < min(ol, kWave); j++) cost += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(seg_o), j));
+    if (ol > kWave && lane == 0) trace<D>(d, 1 - t, next_o, out_o + (size_t)kWave * rs, rs, &cost, &root);
+    if (lane == 0) {
+        st->t_tree = t;
+        st->t_len = tl;
+        st->o_len = ol;
+        st->path_off = off;
+        st->reached = t == 1 ? root_t : root_o;
+        st->cost = cost;
+        st->iters = st->round;
+        st->status = kStatusSolved;
+    }
+}
+
 __device__ __forceinline__ float dd_apply(const PlanDesc &d, float r, bool grow) {
 #pragma clang fp contract(off)
     if (grow) return r == FLT_MAX ? r : r * (1 + d.dd_alpha);                         // pRRTC.cu:456-468
@@ -476,32 +571,11 @@ __device__ int plan_commit(const PlanDesc &d, PlanState *st, int p, PlanControl 
         __threadfence();   // this chunk's radii and nodes before the next chunk reads them
     }
     if (status == kStatusRunning && win_slot >= 0) {
-        // trace both trees (pRRTC.cu:552-582) on lane 0
         if (lane == 0) {
             st->size[0] = sz[0];
             st->size[1] = sz[1];
-            float cost = 0.0f;
-            int root_t, root_o;
-            const int rs = ctrl->path_stride > 0 ? ctrl->path_stride : D;
-            const int tl = trace<D>(d, win_t, win_last, nullptr, rs, &cost, &root_t);
-            const int ol = trace<D>(d, 1 - win_t, win_nn_o, nullptr, rs, &cost, &root_o);
-            const int off = atomic_add_agent(&ctrl->path_rows, tl + ol);
-            if (off + tl + ol > ctrl->path_cap) {
-                st->status = kStatusPathLimit;
-            } else {
-                cost = 0.0f;
-                trace<D>(d, win_t, win_last, paths + (size_t)off * rs, rs, &cost, &root_t);
-                trace<D>(d, 1 - win_t, win_nn_o, paths + (size_t)(off + tl) * rs, rs, &cost, &root_o);
-                st->t_tree = win_t;
-                st->t_len = tl;
-                st->o_len = ol;
-                st->path_off = off;
-                st->reached = win_t == 1 ? root_t : root_o;
-                st->cost = cost;
-                st->iters = st->round;
-                st->status = kStatusSolved;
-            }
         }
+        trace_connection<D>(d, st, ctrl, paths, win_t, win_last, win_nn_o);
         status = __builtin_amdgcn_readfirstlane(lane == 0 ? st->status : 0);
     } else if (lane == 0) {
         st->size[0] = sz[0];

@@ -76,6 +76,34 @@ def test_point_robot_bit_exact(case, seed, gpu, oracle_robots):
     assert path.tobytes() == opath.tobytes()
 
 
+def test_long_connection_trace(gpu, oracle_robots):
+    """A connection whose trees hold long walks (range 0.1 across 15.6 units
+    around three large spheres: a 251-row path), so the commit's trace runs
+    past the 64 rows it keeps in registers: rows, cost and counts bit for bit
+    the oracle's."""
+    from oracle import oracle
+    from prrtc_amd import scenes as S
+    R = oracle_robots["sphere"]
+    sc = S.Scene.build(spheres=[S.sphere((0, 0, 0), 3.0), S.sphere((2.5, -2.5, 2.0), 1.5),
+                                S.sphere((-2.5, 2.5, -2.0), 1.5)])
+    start = np.array([-4.5, -4.5, -4.5], np.float32)
+    goal = np.array([4.5, 4.5, 4.5], np.float32)
+    st = dict(num_new_configs=8, granularity=4, range=0.1, balance=2, tree_ratio=1.0, dynamic_domain=0,
+              max_samples=50000, max_iters=400, seed=3)
+    oracle.set_fused(True)
+    try:
+        rc, ores, opath = R.plan(sc, start, goal[None], st, mode=0)
+    finally:
+        oracle.set_fused(False)
+    assert rc == 0 and ores.solved and len(opath) > 2 * 64 + 2
+    res, path = gpu.solve("sphere", start, goal[None], sc, st)
+    assert res["solved"] and res["iters"] == ores.iters
+    assert (res["start_tree_size"], res["goal_tree_size"]) == (ores.start_tree_size, ores.goal_tree_size)
+    assert res["path_length"] == ores.path_length
+    assert np.float32(res["cost"]).tobytes() == np.float32(ores.cost).tobytes()
+    assert path.tobytes() == opath.tobytes()
+
+
 def _check_path(R, scene, path, start, goals, rng_, g, res):
     assert np.array_equal(path[-1], start)
     assert any(np.array_equal(path[0], gl) for gl in goals)
