@@ -1318,6 +1318,9 @@ int plan_grid(int robot, int waves, long long slots) {
     return (int)std::max<long long>(1, std::min<long long>(slots, (long long)num_cus() * per_cu));
 }
 
+// Path rows read back together with a launch's states.
+constexpr int kHeadRows = 512;
+
 // Solve problems that share one robot in one persistent launch.
 // A robot group in flight: its problems and where its state lives.
 struct GroupRun {
@@ -1326,7 +1329,8 @@ struct GroupRun {
     bool graph_rounds = false;   // graph-replayed round grids instead of one persistent grid
     Workspace *ws = nullptr;
     char *dev = nullptr, *host = nullptr;
-    size_t o_state = 0, o_ctrl = 0, o_paths = 0, h_read = 0;
+    size_t o_state = 0, o_ctrl = 0, o_paths = 0;
+    int head_rows = 0;           // path rows read back with the states (the host image holds them)
 };
 
 // Upload a group, launch its persistent grid and queue the state readback
@@ -1349,59 +1353,64 @@ static void start_group(GroupRun &gr, int path_cap) {
         return v;
     };
     gr.ws = &ws;
-    // --- layout
-    Align dl;
-    const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
-    const size_t o_state = dl.take(sizeof(PlanState) * P);
-    const size_t o_ctrl = dl.take(sizeof(PlanControl));
-    const int words = (P + 63) / 64;
-    const size_t o_mask = dl.take(sizeof(unsigned long long) * 2 * words);   // running | open
+    // --- layout: what is uploaded (descs, masks, control), then what is
+    // read back (control, states, the path rows), then the problems' blocks;
+    // the host image mirrors the first two parts, so each is one copy
     std::vector<BlockLayout> lay(P);
-    std::vector<size_t> o_block(P);
-    int lds_f4 = 0, ext_all = 0;
     int64_t path_rows = 0;
     for (int i = 0; i < P; i++) {
         lay[i] = block_layout(probs[i]);
+        path_rows += std::max(0, path_cap - 2);
+    }
+    Align dl;
+    const size_t o_desc = dl.take(sizeof(PlanDesc) * P);
+    const int words = (P + 63) / 64;
+    const size_t o_mask = dl.take(sizeof(unsigned long long) * 2 * words);   // running | open
+    const size_t o_ctrl = dl.take(sizeof(PlanControl));
+    const size_t o_state = dl.take(sizeof(PlanState) * P);
+    const size_t o_paths = dl.take(sizeof(float) * (size_t)path_rows * D);
+    std::vector<size_t> o_block(P);
+    int lds_f4 = 0, ext_all = 0;
+    for (int i = 0; i < P; i++) {
         lds_f4 = std::max(lds_f4, (lay[i].scene_floats - 8) / 4);
         ext_all = std::max(ext_all, lay[i].max_ext);
         o_block[i] = dl.take(lay[i].bytes);
-        path_rows += std::max(0, path_cap - 2);
     }
-    const size_t o_paths = dl.take(sizeof(float) * (size_t)path_rows * D);
     char *dev = ws.need_dev(dl.off);
-    // --- host image of everything that is uploaded: descs + control, then
-    // each problem's scene and init rows
-    const size_t up_bytes = o_mask + sizeof(unsigned long long) * 2 * words;
+    // the first kHeadRows path rows come back with the states: a solve whose
+    // path fits needs no second copy (and no second wait)
+    const int head_rows = (int)std::min<int64_t>(path_rows, kHeadRows);
+    const size_t up_end = o_ctrl + sizeof(PlanControl);
+    const size_t read_end = o_paths + sizeof(float) * (size_t)head_rows * D;
     Align hl;
-    const size_t h_hdr = hl.take(up_bytes);
+    hl.take(read_end);   // the host image of [0, read_end)
     std::vector<size_t> h_block(P);
     for (int i = 0; i < P; i++) h_block[i] = hl.take(lay[i].upload_bytes);
-    const size_t h_read = hl.take(sizeof(PlanState) * P + sizeof(PlanControl));
     const size_t h_run = hl.take(sizeof(unsigned long long) * words);
     char *host = ws.need_host(hl.off);
-    PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + h_hdr + o_desc);
+    PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + o_desc);
     for (int i = 0; i < P; i++) {
         fill_desc(hd[i], probs[i], lay[i], dev + o_block[i], P == 1);
         block_upload_image(probs[i], lay[i], host + h_block[i]);
     }
-    PlanControl *hc = reinterpret_cast<PlanControl *>(host + h_hdr + o_ctrl);
+    PlanControl *hc = reinterpret_cast<PlanControl *>(host + o_ctrl);
     *hc = PlanControl{};
     hc->n_problems = P;
     hc->mask_words = words;
     hc->run_mask = reinterpret_cast<unsigned long long *>(dev + o_mask);
     hc->open_mask = hc->run_mask + words;   // every problem starts with round 0 open
-    unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + h_hdr + o_mask);
+    unsigned long long *hm = reinterpret_cast<unsigned long long *>(host + o_mask);
     for (int w = 0; w < words; w++) hm[w] = hm[words + w] = (w + 1) * 64 <= P ? ~0ull : (1ull << (P - w * 64)) - 1;
     hc->path_cap = (int)std::min<int64_t>(path_rows, INT32_MAX);
     hc->path_stride = D;
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     hc->prio_policy = kPrioOldest;
     hc->crowd_above = P > 1 ? largest([&](int r) { return plan_crowd_above(r, hc->max_nnc); }) : 0;
-    // --- upload (descs + control in one copy; scenes and init rows per
-    // problem): the control block is complete before the copy is queued
+    // --- upload (descs, masks and control in one copy; scenes and init rows
+    // per problem): the control block is complete before the copy is queued
     // (the copy reads the pinned staging asynchronously)
     const hipStream_t s = ws.stream;
-    PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + h_hdr + o_desc, up_bytes - o_desc, hipMemcpyHostToDevice, s));
+    PRRTC_HIP(hipMemcpyAsync(dev + o_desc, host + o_desc, up_end - o_desc, hipMemcpyHostToDevice, s));
     for (int i = 0; i < P; i++)
         PRRTC_HIP(hipMemcpyAsync(dev + o_block[i], host + h_block[i], lay[i].upload_bytes, hipMemcpyHostToDevice, s));
     // --- launch shape: a single problem runs its tasks on waves_per_task
@@ -1427,16 +1436,14 @@ static void start_group(GroupRun &gr, int path_cap) {
                           lds_f4, ext_all, s);
     }
     PRRTC_HIP(hipEventRecord(ws.e1, s));
-    // --- queue the readback of states + control
-    PRRTC_HIP(hipMemcpyAsync(host + h_read, dev + o_state, sizeof(PlanState) * P, hipMemcpyDeviceToHost, s));
-    PRRTC_HIP(hipMemcpyAsync(host + h_read + sizeof(PlanState) * P, dev + o_ctrl, sizeof(PlanControl),
-                             hipMemcpyDeviceToHost, s));
+    // --- queue the readback of control, states and the first path rows
+    PRRTC_HIP(hipMemcpyAsync(host + o_ctrl, dev + o_ctrl, read_end - o_ctrl, hipMemcpyDeviceToHost, s));
     gr.dev = dev;
     gr.host = host;
     gr.o_state = o_state;
     gr.o_ctrl = o_ctrl;
     gr.o_paths = o_paths;
-    gr.h_read = h_read;
+    gr.head_rows = head_rows;
 }
 
 // Wait for a started group, read the used path rows, write its results.
@@ -1451,14 +1458,18 @@ static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int 
     float ms = 0.0f;
     PRRTC_HIP(hipEventElapsedTime(&ms, ws.e0, ws.e1));
     *kernel_ns = std::max(*kernel_ns, (int64_t)(ms * 1e6));   // groups overlap: the longest one
-    const PlanState *hs = reinterpret_cast<const PlanState *>(gr.host + gr.h_read);
-    const PlanControl *hctl = reinterpret_cast<const PlanControl *>(gr.host + gr.h_read + sizeof(PlanState) * P);
+    const PlanState *hs = reinterpret_cast<const PlanState *>(gr.host + gr.o_state);
+    const PlanControl *hctl = reinterpret_cast<const PlanControl *>(gr.host + gr.o_ctrl);
     const int used = std::min(hctl->path_rows, hctl->path_cap);
-    std::vector<float> rows((size_t)used * D);
-    if (used > 0) {
-        PRRTC_HIP(hipMemcpyAsync(rows.data(), gr.dev + gr.o_paths, sizeof(float) * rows.size(),
+    // rows beyond those read back with the states need a copy of their own
+    std::vector<float> more;
+    const float *rows = reinterpret_cast<const float *>(gr.host + gr.o_paths);
+    if (used > gr.head_rows) {
+        more.resize((size_t)used * D);
+        PRRTC_HIP(hipMemcpyAsync(more.data(), gr.dev + gr.o_paths, sizeof(float) * more.size(),
                                  hipMemcpyDeviceToHost, s));
         PRRTC_HIP(hipStreamSynchronize(s));
+        rows = more.data();
     }
     // --- results, path goal -> start (pRRTC.cu:715-746)
     for (int i = 0; i < P; i++) {
@@ -1477,7 +1488,7 @@ static void finish_group(GroupRun &gr, prrtc_result *results, float *paths, int 
         r.path_nodes = 0;
         r.path_length = 0;
         if (!r.solved) continue;
-        const int row = assemble_path(hp, ps, rows.data(), D, path_cap, paths + (size_t)hp.index * path_stride);
+        const int row = assemble_path(hp, ps, rows, D, path_cap, paths + (size_t)hp.index * path_stride);
         if (row < 0) {
             r.solved = 0;
             r.status = PRRTC_EOVERFLOW;
