@@ -67,7 +67,24 @@ struct Obstacles {
         return o;
     }
 
+    // The same slice for parts = 2^shift, by shifts (cheap enough to take at
+    // every obstacle loop instead of holding a sliced copy in registers).
+    __device__ __forceinline__ Obstacles slice_pow2(int part, int shift) const {
+        Obstacles o = *this;
+        cut2(o.sph, o.n_sph, 1, part, shift);
+        cut2(o.cap, o.n_cap, 2, part, shift);
+        cut2(o.zcap, o.n_zcap, 2, part, shift);
+        cut2(o.cub, o.n_cub, 4, part, shift);
+        cut2(o.zcub, o.n_zcub, 4, part, shift);
+        return o;
+    }
+
 private:
+    __device__ static void cut2(const float4 *&p, int &n, int stride, int part, int shift) {
+        const int lo = (n * part) >> shift, hi = (n * (part + 1)) >> shift;
+        p += (size_t)lo * stride;
+        n = hi - lo;
+    }
     __device__ static void cut(const float4 *&p, int &n, int stride, int part, int parts) {
         const int lo = n * part / parts, hi = n * (part + 1) / parts;
         p += (size_t)lo * stride;

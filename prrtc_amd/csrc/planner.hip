@@ -35,7 +35,7 @@ __device__ __forceinline__ int across_waves(int v, bool want_min, int *buf, int 
     tm.sync();
     int r = b[0];
     for (int w = 1; w < W; w++) r = want_min ? min(r, b[w]) : (r | b[w]);
-    return r;
+    return __builtin_amdgcn_readfirstlane(r);   // alike on every lane: keeps the pass loop's exits scalar
 }
 
 // Check context of a planner task.  The task's waves validate the same 64
@@ -71,6 +71,9 @@ struct TaskCtx : CheckCtx {
     int pass;         // index of the current pass within the task
     int waves;        // waves sharing this pass's poses (1: the wave has poses of its own)
     int gbase;        // first wave of those
+    // this wave's share of the obstacles: part `part` of 2^shift (obs holds
+    // the whole staged scene; the share is cut at every obstacle loop)
+    int part, shift;
     bool shared_stop; // the task runs on several waves: stop keys go through LDS
     unsigned mine12;  // bit i: self groups j with j mod 12 == i are this wave's (12 = lcm of 1..4)
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
@@ -90,13 +93,13 @@ struct TaskCtx : CheckCtx {
     template <int N, bool EARLY>
     __device__ __forceinline__ void env_multi(const float (&x)[N], const float (&y)[N], const float (&z)[N],
                                               const float (&r)[N], bool (&hit)[N]) const {
-        prrtc::env_multi<N, EARLY, true>(obs, x, y, z, r, hit);
+        prrtc::env_multi<N, EARLY, true>(obs.slice_pow2(part, shift), x, y, z, r, hit);
     }
     template <int N, bool EARLY>
     __device__ __forceinline__ bool env_any(const float (&x)[N], const float (&y)[N], const float (&z)[N],
                                             const float (&r)[N]) const {
         bool h[1];
-        prrtc::env_multi<N, EARLY, true, false, true>(obs, x, y, z, r, h);
+        prrtc::env_multi<N, EARLY, true, false, true>(obs.slice_pow2(part, shift), x, y, z, r, h);
         return h[0];
     }
     template <int N, bool EARLY>
@@ -104,7 +107,7 @@ struct TaskCtx : CheckCtx {
                                                     const float (&r)[N], float gx, float gy, float gz,
                                                     float gr) const {
         bool h[1];
-        prrtc::env_multi<N, EARLY, true, true, true>(obs, x, y, z, r, h, gx, gy, gz, gr);
+        prrtc::env_multi<N, EARLY, true, true, true>(obs.slice_pow2(part, shift), x, y, z, r, h, gx, gy, gz, gr);
         return h[0];
     }
     // CheckCtx::stop in path mode (a planner task is never in pose mode),
@@ -144,8 +147,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     // a private copy: its fields stay in registers across the check sequence
     // (through the caller's reference they would be reloaded after every
     // store to the pass flags)
-    TaskCtx ctx = ctx_in;
-    const Obstacles full = ctx_in.obs;   // the whole staged scene
+    TaskCtx ctx = ctx_in;   // ctx.obs: the whole staged scene
     ctx.shared_stop = tm.waves > 1;
     // A pass puts 64 poses on each of G pose groups of the task's waves (G =
     // 1, 2 or all of them).  A group's waves test the same poses and split
@@ -157,13 +159,19 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     // lanes at or above its key on all of them.
     int groups = 0;   // G of the current shape
     int gw = 0;       // this wave's pose group
+    // (task waves are 1, 2 or 4, so a group is 2^k waves)
     auto shape = [&](int G) {
+        // G derives from values every lane holds alike (the chain length),
+        // which the compiler cannot tell: made wave-uniform here, the share
+        // stays scalar, and with it the obstacle loops and group branches
+        G = __builtin_amdgcn_readfirstlane(G);
         if (G == groups) return;
         groups = G;
         const int gs = tm.waves / G;          // waves per group
         gw = tm.wave / gs;
         const int wi = tm.wave - gw * gs;     // this wave's index in its group
-        ctx.obs = gs == 1 ? full : full.slice(wi, gs);
+        ctx.part = wi;
+        ctx.shift = gs == 4 ? 2 : gs == 2 ? 1 : 0;
         ctx.waves = gs;
         ctx.gbase = gw * gs;
         unsigned m = 0;
@@ -183,7 +191,9 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
     const int lane = lane_id();
     const int iter = st->round;
     const int n0 = st->size[0], n1 = st->size[1];
-    const int g = d.g;
+    // the pass loop's bounds, wave-uniform to the compiler too: the group
+    // shares and the ctx fields carried through the loop stay scalar
+    const int g = __builtin_amdgcn_readfirstlane(d.g);
     const float gf = (float)g;
     int *meta = d.meta + (size_t)s * 8;
     float *stage = d.stage + (size_t)s * (d.max_ext + 1) * D;
@@ -302,8 +312,8 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
             const float *on = d.nodes + (size_t)o * D * d.cap;
             nn_o = nearest<D>(on, d.cap, o == 0 ? n0 : n1, cfg, bsq, red, tm);
             const float dist_o = sqrtf(bsq);
-            n_ext = (unsigned)ceilf(dist_o / d.range);
-            n_chain = (int)min(n_ext, (unsigned)d.max_ext);
+            n_ext = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)ceilf(dist_o / d.range));
+            n_chain = __builtin_amdgcn_readfirstlane((int)min(n_ext, (unsigned)d.max_ext));
             if (tid < D) {
                 const float vec = (on[(size_t)tid * d.cap + nn_o] - chain[tid]) / (float)n_ext;
                 vecv[tid] = vec;
@@ -1098,18 +1108,23 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
 }
 
 // Chain pass from which a task's waves take distinct poses (negative:
-// never).  A problem alone whose round fits one task per CU waits for its
-// slowest task, so every chain pass goes wide; otherwise the first chain pass
-// shares its 64 poses (many chains collide in their first sub-edges, and a
-// shared pass ends soonest there; with more tasks than CUs the extra poses
-// of a wide first pass cost throughput).  PRRTC_WIDE_FROM overrides.
+// never).  A problem alone whose round leaves half the CUs free waits for
+// its slowest task, so every chain pass goes wide; one whose round fills the
+// machine is bound by throughput, and the poses a wide pass adds (past a
+// chain's first collision, on waves without poses) cost more than its
+// shorter passes save, so none goes wide.  In a batch, tasks share their
+// waves only while few problems run (the latency-bound tail); there the
+// first chain pass shares its 64 poses (many chains collide in their first
+// sub-edges, where a shared pass ends soonest) and the later ones go wide.
+// PRRTC_WIDE_FROM overrides.
 static int wide_from(int nnc, bool alone) {
     static const int env = [] {
         const char *e = std::getenv("PRRTC_WIDE_FROM");
         return e ? std::atoi(e) : INT_MIN;
     }();
     if (env != INT_MIN) return env;
-    return alone && nnc <= num_cus() ? 0 : 1;
+    if (alone) return 2 * nnc <= num_cus() ? 0 : -1;
+    return 1;
 }
 
 // Whether a steered edge of more than 64 poses spreads them over pose groups
@@ -1269,18 +1284,21 @@ static int env_int(const char *name, int lo, int hi) {
     return n >= lo && n <= hi ? n : 0;
 }
 
+// waves of a task: 1, 2 or 4 (a task's obstacle shares are cut by shifts)
+static int pow2_waves(int w) { return w == 3 ? 0 : w; }
+
 static int tuned(const std::atomic<int> *table, int robot, int env, int dflt) {
     const int v = robot >= 0 && robot < kNumRobots ? table[robot].load(std::memory_order_relaxed) : 0;
     return v > 0 ? v : (env > 0 ? env : dflt);
 }
 
 int plan_task_waves(int robot) {
-    static const int env = env_int("PRRTC_TASK_WAVES", 1, kMaxTaskWaves);
+    static const int env = pow2_waves(env_int("PRRTC_TASK_WAVES", 1, kMaxTaskWaves));
     return tuned(g_task_waves, robot, env, kDefaultTaskWaves);
 }
 
 int plan_batch_waves(int robot) {
-    static const int env = env_int("PRRTC_BATCH_WAVES", 1, kMaxTaskWaves);
+    static const int env = pow2_waves(env_int("PRRTC_BATCH_WAVES", 1, kMaxTaskWaves));
     return tuned(g_batch_waves, robot, env, kDefaultBatchWaves);
 }
 
@@ -1602,7 +1620,8 @@ int prrtc_solve(int robot, const float *start, const float *goals, int n_goals, 
 int prrtc_set_tuning(int robot, const prrtc_tuning *t) {
     auto bad = [](int v, int hi) { return v < 0 || v > hi; };
     if (!t || robot < -1 || robot >= kNumRobots || bad(t->waves_per_cu, kMaxWavesPerCu) ||
-        bad(t->waves_per_task, kMaxTaskWaves) || bad(t->batch_waves_per_task, kMaxTaskWaves)) {
+        bad(t->waves_per_task, kMaxTaskWaves) || bad(t->batch_waves_per_task, kMaxTaskWaves) ||
+        t->waves_per_task == 3 || t->batch_waves_per_task == 3) {   // a task's waves: 1, 2 or 4
         set_error("prrtc_set_tuning: invalid argument");
         return PRRTC_EINVAL;
     }
