@@ -107,7 +107,7 @@ int solve_noalloc(const HostProblem &hp, prrtc_result *out, float *path, int pat
     PRRTC_HIP(hipMemcpyAsync(blk + L.init + sizeof(float) * D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D,
                              hipMemcpyHostToDevice, s));
     PlanDesc desc;
-    fill_desc(desc, hp, L, blk, true);
+    fill_desc(desc, hp, L, blk);
     PlanControl ctrl{};
     ctrl.n_problems = 1;
     ctrl.mask_words = 1;
@@ -116,6 +116,7 @@ int solve_noalloc(const HostProblem &hp, prrtc_result *out, float *path, int pat
     ctrl.path_cap = (int)std::min<size_t>(path_rows, INT32_MAX);
     ctrl.max_nnc = hp.st->num_new_configs;
     ctrl.prio_policy = kPrioOldest;
+    ctrl.wide_slots = plan_wide_slots();
     ctrl.crowd_above = 0;
     const unsigned long long masks[2] = {1ull, 1ull};   // problem 0 runs, round 1 open
     PRRTC_HIP(hipMemcpyAsync(dev + o_desc, &desc, sizeof desc, hipMemcpyHostToDevice, s));

@@ -6,7 +6,6 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
-#include <climits>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -75,6 +74,7 @@ struct TaskCtx : CheckCtx {
     // the whole staged scene; the share is cut at every obstacle loop)
     int part, shift;
     bool shared_stop; // the task runs on several waves: stop keys go through LDS
+    bool wide_ok;     // its chain passes may give each wave poses of its own
     unsigned mine12;  // bit i: self groups j with j mod 12 == i are this wave's (12 = lcm of 1..4)
     __device__ __forceinline__ void share_gates(uint64_t &gate) {
         const int W = waves;
@@ -179,6 +179,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
         // uniform over the wave: the group branches stay scalar branches
         ctx.mine12 = (unsigned)__builtin_amdgcn_readfirstlane((int)m);
     };
+    const bool wide_ok = __builtin_amdgcn_readfirstlane((int)ctx_in.wide_ok) != 0 && d.wide_from >= 0;
     // pose groups for n poses: one pass where the waves allow it
     auto groups_for = [&](int n) {
         const int want = (n + kWave - 1) / kWave;
@@ -269,7 +270,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
             } else {
                 // from chain pass wide_from on, the remaining poses spread
                 // over pose groups: group gw takes poses base + 64 gw + lane
-                shape(d.wide_from >= 0 && cpass >= d.wide_from ? groups_for(total - base) : 1);
+                shape(wide_ok && cpass >= d.wide_from ? groups_for(total - base) : 1);
                 const int j = base + gw * kWave + lane;
                 const int jj = j < total ? j : total - 1;
                 const int i = jj / g, k = jj % g;
@@ -719,6 +720,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
     while (true) {
         if (leader) {
             int p_out = -1, s_out = -1, n_out = 1, hot = 0, crowd = 0;   // p_out < 0: the workgroup leaves
+            int run_now = 0;   // running problems at the claim
             // Claim up to k consecutive slots of problem q if q runs and has
             // unclaimed slots in this round; returns the first slot (-1: none)
             // and their number in `got`.  A round the mask scan saw open is
@@ -764,6 +766,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 const int hp = __builtin_amdgcn_readfirstlane(hk ? (int)(hk & 0xffffffffu) : -1);
                 const int ph = hp >= 0 && hp < P ? hp : -1;
                 const MaskScan ms = scan_masks(ctrl, words, cursor, ph);
+                run_now = ms.running;
                 // the running problems can use at most running * nnc
                 // workgroups: the rest leave, so a few long problems do not
                 // share their counters with the whole grid
@@ -843,6 +846,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
                 ctl[3] = prio;
                 ctl[4] = crowd;
                 ctl[5] = halt;
+                ctl[6] = (long long)run_now * max_nnc <= ctrl->wide_slots;
             }
         }
         __syncthreads();
@@ -880,6 +884,7 @@ void k_plan(const PlanDesc *__restrict__ descs, PlanState *states, PlanControl *
             __threadfence();
         } else if (runs) {
             ctx.obs = scene_obs;
+            ctx.wide_ok = !crowd && ctl[6] != 0;
             float *a = areas + (crowd ? (size_t)w * area : 0);
             run_task<Robot>(d, st, crowd ? s + w : s, ctx, a, lds_ext, crowd ? team_wave() : team_workgroup());
             __threadfence();
@@ -961,6 +966,7 @@ void k_round_tasks(const PlanDesc *__restrict__ descs, PlanState *states, int ma
     ctx.independent = false;
     ctx.gbuf = gbuf;
     bind_scene<true>(d.scene, d.scene_f4, lds, ctx.obs);
+    ctx.wide_ok = false;   // a grid of every running problem's slots: throughput-bound
     plan_task<Robot>(d, st, s, ctx, a, a + (lds_ext + 1) * D, a + (lds_ext + 5) * D,
                      reinterpret_cast<int *>(a + (lds_ext + 5) * D + 2 * kMaxTaskWaves), team_workgroup());
 }
@@ -1107,25 +1113,32 @@ void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host)
     std::memcpy(ini + D, hp.goals, sizeof(float) * (size_t)hp.n_goals * D);
 }
 
-// Chain pass from which a task's waves take distinct poses (negative:
-// never).  A problem alone whose round leaves half the CUs free waits for
-// its slowest task, so every chain pass goes wide; one whose round fills the
-// machine is bound by throughput, and the poses a wide pass adds (past a
-// chain's first collision, on waves without poses) cost more than its
-// shorter passes save, so none goes wide.  In a batch, tasks share their
-// waves only while few problems run (the latency-bound tail); there the
-// first chain pass shares its 64 poses (many chains collide in their first
-// sub-edges, where a shared pass ends soonest) and the later ones go wide.
-// PRRTC_WIDE_FROM overrides.
-static int wide_from(int nnc, bool alone) {
+// Chain pass from which a task's waves may take distinct poses (negative:
+// never; PRRTC_WIDE_FROM overrides the robot's default).  Whether they do is
+// decided per task (PlanControl::wide_slots).  A wide pass saves the
+// forward kinematics the waves of a shared pass compute alike and the gate
+// exchange; Fetch's checks are dominated by its bounded leaf groups rather
+// than its kinematics, and its solves measured slower with wide passes (48
+// C2 problems alone at nnc 128: 748 -> 923 ms of kernel; the mix's Fetch
+// problems 197 -> 214 ms), Panda's and Baxter's faster (C0 16.5 -> 14.6 ms,
+// the C1 plans 1582 -> 1461 ms, the mix's Panda 2236 -> 2110 and Baxter
+// 1016 -> 965 ms).
+static int wide_from(int robot) {
     static const int env = [] {
         const char *e = std::getenv("PRRTC_WIDE_FROM");
-        return e ? std::atoi(e) : INT_MIN;
+        return e ? std::atoi(e) : INT32_MIN;
     }();
-    if (env != INT_MIN) return env;
-    if (alone) return 2 * nnc <= num_cus() ? 0 : -1;
-    return 1;
+    if (env != INT32_MIN) return env;
+    return robot == PRRTC_ROBOT_FETCH ? -1 : 0;
 }
+
+// Chain passes go wide only while the running problems' rounds leave half
+// the CUs free: such rounds wait for their slowest task.  A round that fills
+// the machine is bound by throughput, and the poses a wide pass adds (past a
+// chain's first collision, on waves without poses) cost more than its
+// shorter passes save (the 256-slot C2 plain solves 15% slower, the C2
+// batch's tail 13%).
+int plan_wide_slots() { return num_cus() / 2; }
 
 // Whether a steered edge of more than 64 poses spreads them over pose groups
 // (PRRTC_WIDE_EDGE overrides).
@@ -1137,9 +1150,9 @@ static int wide_edge() {
     return v;
 }
 
-void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev, bool alone) {
+void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev) {
     d = PlanDesc{};
-    d.wide_from = wide_from(hp.st->num_new_configs, alone);
+    d.wide_from = wide_from(hp.robot);
     d.wide_edge = wide_edge();
     d.nnc = hp.st->num_new_configs;
     d.g = hp.st->granularity;
@@ -1408,7 +1421,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     char *host = ws.need_host(hl.off);
     PlanDesc *hd = reinterpret_cast<PlanDesc *>(host + o_desc);
     for (int i = 0; i < P; i++) {
-        fill_desc(hd[i], probs[i], lay[i], dev + o_block[i], P == 1);
+        fill_desc(hd[i], probs[i], lay[i], dev + o_block[i]);
         block_upload_image(probs[i], lay[i], host + h_block[i]);
     }
     PlanControl *hc = reinterpret_cast<PlanControl *>(host + o_ctrl);
@@ -1423,6 +1436,7 @@ static void start_group(GroupRun &gr, int path_cap) {
     hc->path_stride = D;
     for (int i = 0; i < P; i++) hc->max_nnc = std::max(hc->max_nnc, probs[i].st->num_new_configs);
     hc->prio_policy = kPrioOldest;
+    hc->wide_slots = plan_wide_slots();
     hc->crowd_above = P > 1 ? largest([&](int r) { return plan_crowd_above(r, hc->max_nnc); }) : 0;
     // --- upload (descs, masks and control in one copy; scenes and init rows
     // per problem): the control block is complete before the copy is queued

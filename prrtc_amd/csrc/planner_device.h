@@ -83,7 +83,10 @@ struct PlanControl {
     // floats between path rows in the path output: the largest DOF of a
     // launch whose problems have different robots (0: the robot's DOF)
     int path_stride;
-    int pad1;
+    // a task's connect-chain passes may give each wave poses of its own
+    // while (running problems) x max_nnc is at most this (0: never): the
+    // rounds are then bound by their slowest task, not by throughput
+    int wide_slots;
     unsigned long long t0;          // launch start (s_memrealtime ticks), set by the first workgroup
     unsigned long long slice_ticks; // 0: run until every problem stops; else leave after this long
     unsigned long long *run_mask;   // bit p set while problem p runs; cleared by its last commit

@@ -45,8 +45,9 @@ BlockLayout block_layout(const HostProblem &hp);
 // Writes the packed scene and init rows to `host` (layout.upload_bytes).
 void block_upload_image(const HostProblem &hp, const BlockLayout &L, char *host);
 // Fills a PlanDesc whose buffers live in the device block at `dev`.
-// `alone`: the launch holds this problem only (it picks the chain pass shape)
-void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev, bool alone);
+void fill_desc(PlanDesc &d, const HostProblem &hp, const BlockLayout &L, char *dev);
+// PlanControl::wide_slots of a launch on this device
+int plan_wide_slots();
 
 // Launch shapes of a robot's persistent planner grids: the value set by
 // prrtc_set_tuning, else an environment variable, else the default.

@@ -323,7 +323,7 @@ void ServerImpl::launch(RobotClass &c) {
             block_upload_image(hp, rq->lay, stg + so);
             PRRTC_HIP(hipMemcpyAsync(arena_dev + rq->off, stg + so, rq->lay.upload_bytes, hipMemcpyHostToDevice, s));
             so += (rq->lay.upload_bytes + 255) & ~size_t(255);
-            fill_desc(c.h_desc[rq->entry], hp, rq->lay, arena_dev + rq->off, false);
+            fill_desc(c.h_desc[rq->entry], hp, rq->lay, arena_dev + rq->off);
             c.h_list[i] = rq->entry;
             lo = std::min(lo, rq->entry);
             hi = std::max(hi, rq->entry);
@@ -364,6 +364,7 @@ void ServerImpl::launch(RobotClass &c) {
     hc.slice_ticks = (unsigned long long)cfg.slice_us * ticks_per_us;
     hc.mask_words = words;
     hc.prio_policy = kPrioYoungest;
+    hc.wide_slots = plan_wide_slots();
     hc.crowd_above = plan_crowd_above(c.robot, max_nnc);
     hc.run_mask = c.d_mask;
     hc.open_mask = c.d_mask + W;
