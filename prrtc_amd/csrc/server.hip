@@ -340,6 +340,13 @@ void ServerImpl::launch(RobotClass &c) {
     const int words = (c.high + 63) / 64;
     const int W = c.mask_words;
     for (int w = 0; w < 3 * W; w++) c.h_mask[w] = 0ull;
+    // while a stopped request is resident, every other request is served as
+    // aged, so the stopped ones come first: on a server past its capacity the
+    // young requests alone could otherwise keep a stopped round from being
+    // claimed for many slices, and its result from coming back near its
+    // deadline
+    bool any_stopped = false;
+    for (int e = 0; e < c.high; e++) any_stopped |= c.entries[e] && c.entries[e]->stop_reason;
     for (int e = 0; e < c.high; e++) {
         const Request *rq = c.entries[e];
         if (!rq) continue;
@@ -350,7 +357,8 @@ void ServerImpl::launch(RobotClass &c) {
         // a stopped request counts as young: its slots (skipped tasks) are
         // claimed first, so the round that ends it commits at once instead of
         // waiting behind the young requests of a busy server
-        if (rq->round >= cfg.young_rounds && !rq->stop_reason) c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
+        if (!rq->stop_reason && (any_stopped || rq->round >= cfg.young_rounds))
+            c.h_mask[2 * W + (e >> 6)] |= 1ull << (e & 63);
         lds_f4 = std::max(lds_f4, (rq->lay.scene_floats - 8) / 4);
         ext = std::max(ext, rq->lay.max_ext);
         max_nnc = std::max(max_nnc, rq->st.num_new_configs);
