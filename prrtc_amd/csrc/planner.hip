@@ -249,7 +249,7 @@ __device__ __attribute__((always_inline)) void plan_task(const PlanDesc &d, Plan
         tm.sync();
         // phase 0: the steered edge, poses nn + (k+1) delta in interleaved passes;
         // phase 1: the connect chain, pose j on sub-edge j / g at c_i + (j%g + 1) step
-        const int G0 = d.wide_edge ? groups_for(g) : 1;
+        const int G0 = __builtin_amdgcn_readfirstlane(d.wide_edge ? groups_for(g) : 1);
         const int passes = (g + kWave * G0 - 1) / (kWave * G0);
         shape(G0);
         int phase = 0, p = 0, base = 0, total = 0, n_chain = 0, npass = 0, cpass = 0;
