@@ -92,6 +92,7 @@ struct Request {
     int64_t deadline_ns = 0;   // absolute (now_ns clock), 0: none
     bool cancel = false;       // prrtc_server_cancel asked to end it
     int stop_reason = 0;       // PRRTC_ETIMEDOUT / PRRTC_ECANCELED once its stop word is set
+    int stop_slices = 0;       // slices it had run when its stop word was set
     HostProblem hp() const { return HostProblem{robot, start.data(), goals.data(), n_goals, &scene, &st, 0}; }
 };
 
@@ -298,12 +299,16 @@ void ServerImpl::expire_locked(std::vector<std::pair<uint64_t, Finished>> &out) 
         if (rq->stop_reason || rq->entry < 0) continue;
         if (const int why = reason(*rq)) {
             rq->stop_reason = why;
+            rq->stop_slices = rq->slices;
             cls[rq->robot].h_stop[rq->entry] = 1;
             cls[rq->robot].stop_dirty = true;
         }
     }
     stats.pending = (int32_t)queue.size();
 }
+
+// slices a stopped request waits behind young ones before it goes first
+constexpr int kStopSlices = 4;
 
 void ServerImpl::launch(RobotClass &c) {
     c.launched = false;
@@ -340,13 +345,16 @@ void ServerImpl::launch(RobotClass &c) {
     const int words = (c.high + 63) / 64;
     const int W = c.mask_words;
     for (int w = 0; w < 3 * W; w++) c.h_mask[w] = 0ull;
-    // while a stopped request is resident, every other request is served as
-    // aged, so the stopped ones come first: on a server past its capacity the
-    // young requests alone could otherwise keep a stopped round from being
-    // claimed for many slices, and its result from coming back near its
-    // deadline
+    // A stopped request counts as young, which ends it within a few slices
+    // while the server keeps up.  Past its capacity the young requests alone
+    // can keep a stopped round from being claimed for many slices, so once a
+    // stopped request has waited kStopSlices slices every other request is
+    // served as aged until it ends: the stopped rounds are claimed first and
+    // their results come back near their deadlines.
     bool any_stopped = false;
-    for (int e = 0; e < c.high; e++) any_stopped |= c.entries[e] && c.entries[e]->stop_reason;
+    for (int e = 0; e < c.high; e++)
+        any_stopped |= c.entries[e] && c.entries[e]->stop_reason &&
+                       c.entries[e]->slices - c.entries[e]->stop_slices >= kStopSlices;
     for (int e = 0; e < c.high; e++) {
         const Request *rq = c.entries[e];
         if (!rq) continue;
