@@ -55,20 +55,9 @@ struct Obstacles {
         zcub = cub + 4 * n_cub;
     }
 
-    // Slice `part` of `parts`: a contiguous share of every table (the slices
-    // of one table partition it).
-    __device__ Obstacles slice(int part, int parts) const {
-        Obstacles o = *this;
-        cut(o.sph, o.n_sph, 1, part, parts);
-        cut(o.cap, o.n_cap, 2, part, parts);
-        cut(o.zcap, o.n_zcap, 2, part, parts);
-        cut(o.cub, o.n_cub, 4, part, parts);
-        cut(o.zcub, o.n_zcub, 4, part, parts);
-        return o;
-    }
-
-    // The same slice for parts = 2^shift, by shifts (cheap enough to take at
-    // every obstacle loop instead of holding a sliced copy in registers).
+    // Share `part` of 2^shift: a contiguous share of every table (the shares
+    // of one table partition it), cut by shifts (cheap enough to take at every
+    // obstacle loop instead of holding a sliced copy in registers).
     __device__ __forceinline__ Obstacles slice_pow2(int part, int shift) const {
         Obstacles o = *this;
         cut2(o.sph, o.n_sph, 1, part, shift);
@@ -82,11 +71,6 @@ struct Obstacles {
 private:
     __device__ static void cut2(const float4 *&p, int &n, int stride, int part, int shift) {
         const int lo = (n * part) >> shift, hi = (n * (part + 1)) >> shift;
-        p += (size_t)lo * stride;
-        n = hi - lo;
-    }
-    __device__ static void cut(const float4 *&p, int &n, int stride, int part, int parts) {
-        const int lo = n * part / parts, hi = n * (part + 1) / parts;
         p += (size_t)lo * stride;
         n = hi - lo;
     }
